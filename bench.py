@@ -1,0 +1,217 @@
+"""Benchmark: 3D level-set fast sweeping at 257^3 on one MI355X.
+
+Workload: grid nz = nx = ny = 257, h = 20 m, 8 flat layers at seeded random
+depths with velocities 1500 -> 6500 m/s increasing with depth, times a
+Gaussian-smoothed (sigma = 5 cells) +-5 % random perturbation; one source
+at a seeded random position; 8 sweep directions x 2 iterations after the
+initialization sweeps (the reference's default nsweep = 2).
+
+One step = one complete single-source solve (fteik_solver3d_solveSourceLSM)
+with the model and source already resident on the GPU.  The FP64 path
+(reference semantics, bitwise equal to the oracle) and the mixed path (FP32
+Gauss-Seidel iterations, then FP64 correction iterations to convergence) are
+each timed over K steps after W warm-up steps.
+
+Output: one JSON line.  value = node updates per second of the FP64 solve
+(updates as the reference counts them: initialization masks plus
+nsweep x 8 x (n-1)^3), in Gupdates/s.  The mixed path's rate on the same
+update count, its speed-up and its error against the FP64 field are
+reported beside it.  roofline: the FP64 Gauss-Seidel sweep kernel, timed
+with hipEvents on the library's stream; cpu_baseline: the oracle's OpenMP
+level-synchronous FP64 solve of the same workload on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from ctypes import byref, c_double, c_int
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N, H = 257, 20.0
+BYTES_PER_UPDATE = {"f64": 24.0, "f32": 12.0}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep3d_257_traffic.json")
+
+
+def layered_model(n=N, h=H, seed=2025):
+    """Cell velocities (natural order, z fastest) and a source position."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    nc = n - 1
+    depth = nc * h
+    interfaces = np.sort(rng.uniform(0.05 * depth, 0.95 * depth, 7))
+    vlayer = np.linspace(1500.0, 6500.0, 8)
+    zc = (np.arange(nc) + 0.5) * h
+    v1d = vlayer[np.searchsorted(interfaces, zc)]
+    g = gaussian_filter(rng.standard_normal((nc, nc, nc)), sigma=5.0,
+                        mode="reflect")
+    g /= np.abs(g).max()
+    vel = v1d[None, None, :] * (1.0 + 0.05 * g)
+    src = rng.uniform(2.0 * h, (n - 3) * h, 3)       # (z, x, y)
+    return np.ascontiguousarray(vel.ravel()), src
+
+
+def hip_sync():
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    return hip.hipDeviceSynchronize()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not "
+              "supported (the solver has no multi-GPU path)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import fteik_amd
+    L = fteik_amd.load_library()
+    vel, (zs, xs, ys) = layered_model()
+    ierr = c_int(1)
+
+    def init(precision):
+        L.fteik_solver3d_initialize(N, N, N, 0.0, 0.0, 0.0, H, H, H, 2, 5.0,
+                                    0.0, 0, byref(ierr))
+        assert ierr.value == 0, "initialize"
+        L.fteik_solver3d_setVelocityModel64f(vel.size, vel.ctypes.data_as(
+            ctypes.POINTER(c_double)), byref(ierr))
+        assert ierr.value == 0, "model"
+        z, x, y = (np.array([v]) for v in (zs, xs, ys))
+        dp = ctypes.POINTER(c_double)
+        L.fteik_solver3d_setSources64f(1, z.ctypes.data_as(dp),
+                                       x.ctypes.data_as(dp),
+                                       y.ctypes.data_as(dp), byref(ierr))
+        assert ierr.value == 0, "sources"
+        L.fteik_solver3d_setPrecision(precision, byref(ierr))
+        assert ierr.value == 0, "precision"
+
+    def run(k):
+        for _ in range(k):
+            L.fteik_solver3d_solveSourceLSM(1, byref(ierr))
+            if ierr.value != 0:
+                raise RuntimeError("solve failed")
+
+    def info():
+        ms, upd, corr = c_double(), c_double(), c_int()
+        e = c_int()
+        L.fteik_solver3d_getLastSolveInfo(byref(ms), byref(corr), byref(upd),
+                                          byref(e))
+        return ms.value, corr.value, upd.value
+
+    def field():
+        tt = np.zeros(N ** 3)
+        L.fteik_solver3d_getTravelTimeField64f(tt.size, 0, tt.ctypes.data_as(
+            ctypes.POINTER(c_double)), byref(ierr))
+        assert ierr.value == 0
+        return tt
+
+    def sweep_ms(precision, nrep=4):
+        ms = c_double()
+        L.fteik_amd_timeSweeps3d(precision, nrep, byref(ms), byref(ierr))
+        assert ierr.value == 0, "timeSweeps3d"
+        return ms.value
+
+    # ---------------------------------------------------------- FP64 path
+    init(0)
+    run(args.warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    run(args.steps)
+    hip_sync()
+    t64 = (time.perf_counter() - t0) / args.steps
+    _, _, upd64 = info()
+    tt64 = field()
+    print("bench: fp64 solve %.3f ms" % (t64 * 1e3), file=sys.stderr,
+          flush=True)
+    ms_launch64 = sweep_ms(0)
+
+    # --------------------------------------------------------- mixed path
+    init(1)
+    run(args.warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    run(args.steps)
+    hip_sync()
+    tmix = (time.perf_counter() - t0) / args.steps
+    _, corr, updmix = info()
+    ttm = field()
+    rel = np.abs(ttm - tt64) / np.maximum(tt64, 1e-12)
+    print("bench: mixed solve %.3f ms, %d FP64 correction iterations"
+          % (tmix * 1e3, corr), file=sys.stderr, flush=True)
+    ms_launch32 = sweep_ms(1)
+
+    units = float((N - 1) ** 3)          # updates per Gauss-Seidel launch
+    ach64 = BYTES_PER_UPDATE["f64"] * units / (ms_launch64 * 1e-3) / 1e9
+    traffic = None
+    if os.path.isfile(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("fp64_bytes_per_launch")
+
+    # ------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(cores)
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        from oracle import oracle as orc
+        slow = orc.cell_slowness(vel, N, N, N)
+        src = orc.source_locate((N, N, N, H, H, H, 0.0, 0.0, 0.0), zs, xs, ys,
+                                slow)
+        t0 = time.perf_counter()
+        ref = orc.solve3d(slow, N, N, N, H, H, H, src, 2, 5.0, schedule=1)
+        tcpu = time.perf_counter() - t0
+        cpu = {"value": upd64 / tcpu / 1e9, "unit": "Gupdates/s",
+               "cores": cores, "kind": "port",
+               "sample": "one complete FP64 solve of the same 257^3 "
+                         "workload (oracle, OpenMP level-synchronous)",
+               "seconds": tcpu,
+               "fp64_gpu_equals_oracle": bool(np.array_equal(ref, tt64))}
+
+    out = {
+        "metric": "Gupdates/s at 257^3 FP64 vs FP32-mixed",
+        "value": upd64 / t64 / 1e9,
+        "unit": "Gupdates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D single-source level-set sweeps, 257^3, "
+                               "h=20 m, 8 layers 1500-6500 m/s +-5% smoothed "
+                               "perturbation, nsweep=2",
+                   "grid": [N, N, N], "h": H, "nsweep": 2, "eps": 5.0,
+                   "source_zxy": [zs, xs, ys]},
+        "mixed": {"value": upd64 / tmix / 1e9, "unit": "Gupdates/s",
+                  "ms_per_step": tmix * 1e3, "speedup_vs_fp64": t64 / tmix,
+                  "fp64_correction_iterations": corr,
+                  "updates_performed": updmix,
+                  "max_rel_err_vs_fp64": float(rel.max()),
+                  "median_rel_err_vs_fp64": float(np.median(rel)),
+                  "fp32_sweep_ms_per_launch": ms_launch32,
+                  "fp32_sweep_GBps": BYTES_PER_UPDATE["f32"] * units
+                  / (ms_launch32 * 1e-3) / 1e9},
+        "roofline": {"kernel": "sweep3d_pencil_kernel<double,false>",
+                     "bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "ms_per_launch": ms_launch64,
+                     "bytes_per_update": BYTES_PER_UPDATE["f64"],
+                     "updates_per_launch": units},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

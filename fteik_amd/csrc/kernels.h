@@ -1,0 +1,117 @@
+// kernels.h -- internal launcher interface between the host C-ABI layer
+// (capi.cpp) and the HIP kernels.  Not installed; plain C structs only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fteik {
+
+struct SourceGeom {
+    double dz, dx, dy, eps;
+    double zsa, xsa, ysa;
+    int zsi, xsi, ysi;
+    int nz, nx, ny;
+};
+
+struct SweepLaunch {
+    void* tt;                    // double* or float* field (natural order)
+    const void* slow;            // double* or float* slowness
+    const void* cst;             // Consts<double|float>*
+    unsigned long long* flags;   // per-block progress flags
+    int* err;                    // device error flag
+    double* maxdelta;            // optional convergence monitor (bits)
+    unsigned long long serial;   // launch serial for the flags
+    int max_flags;
+    int nz, nx, ny;
+    int zsi, xsi, ysi;
+    int fp32;                    // 0: double, 1: float
+    int schedule;                // 0: pipelined column blocks, 1: per level
+    int cooperative;
+};
+
+} // namespace fteik
+
+extern "C" {
+// sweep3d.hip
+int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
+                    hipStream_t stream);
+int fteik_k_pencil_fits(int nx, int ny, int fp32);
+int fteik_k_source3d_init(void* cst, void* tt, const void* slow,
+                          const fteik::SourceGeom* g, long ngrd, int fp32,
+                          hipStream_t stream);
+int fteik_k_consts_to_float(const void* cd, void* cf, hipStream_t stream);
+
+// util.hip
+int fteik_k_slowness_from_cells(double* slow, const double* vel, int nz,
+                                int nx, int ny, int is3d, int order,
+                                hipStream_t stream);
+int fteik_k_slowness_from_nodes(double* slow, const double* vel, int nz,
+                                int nx, int ny, int is3d, int order,
+                                hipStream_t stream);
+int fteik_k_slowness_from_velocity32(double* slow, const float* vel, long n,
+                                     hipStream_t stream);
+int fteik_k_field_reorder(double* out, const double* tt, int nz, int nx,
+                          int ny, int is3d, int order, hipStream_t stream);
+int fteik_k_field_cells(double* out, const double* tt, int nz, int nx,
+                        int ny, int is3d, int order, hipStream_t stream);
+int fteik_k_receivers(double* ttr, const double* tt, const int* ri,
+                      const double* rd, int nrec, int nz, int nx, double dz,
+                      double dx, double dy, int is3d, hipStream_t stream);
+int fteik_k_d2f(float* out, const double* in, long n, hipStream_t stream);
+int fteik_k_f2d_up(double* out, const float* in, long n, hipStream_t stream);
+int fteik_k_analytic(double* tt_xfast, int nz, int nx, int ny, double dz,
+                     double dx, double dy, double z0, double x0, double y0,
+                     double zs, double xs, double ys, double vconst,
+                     double vtop, double vbot, int gradient,
+                     hipStream_t stream);
+int fteik_k_analytic_points(double* out, const double* z, const double* x,
+                            const double* y, int n, int nz, double dz,
+                            double z0, double zs, double xs, double ys,
+                            double vconst, double vtop, double vbot,
+                            int gradient, hipStream_t stream);
+int fteik_k_analytic_reorder(double* out, const double* tt_xfast, int nz,
+                             int nx, int ny, int order, hipStream_t stream);
+
+// sweep2d.hip
+int fteik_k_solve2d(double* tt, const double* slow, int nz, int nx,
+                    double dz, double dx, double eps, double zsa, double xsa,
+                    int zsi, int xsi, int nsweep, int nsrc_batch,
+                    int ld_tt, const double* zsa_v, const double* xsa_v,
+                    const int* zsi_v, const int* xsi_v, hipStream_t stream);
+
+// locate.hip
+int fteik_k_tables_d2f(float* tables, const double* tt, int ngrd,
+                       hipStream_t stream);
+int fteik_k_locate(const float* tables, int ntf, int ngrd, long ldgrd,
+                   const float* obs, const float* wt, const unsigned char* skip,
+                   int lntf, float xnorm, float* t0work, float* objwork,
+                   int accumulate, unsigned long long* best,
+                   hipStream_t stream);
+int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
+                         const float* obs, const float* wt,
+                         const unsigned char* skip, const float* xnorm,
+                         int lntf, int nev, unsigned long long* best,
+                         float* t0best, hipStream_t stream);
+
+// stencil_eval.hip (diagnostics: device stencils on given node inputs)
+int fteik_k_eval3d(const double* in, const int* node, const void* cst,
+                   int n, double* out_noinit, double* out_init,
+                   hipStream_t stream);
+}
+
+extern "C" {
+// util.hip (continued)
+int fteik_k_min_merge_f(double* tt, const float* tt32, long n,
+                        hipStream_t stream);
+int fteik_k_graph_ijkv(int* ijkv, const int* levelPtr, int nz, int nx, int ny,
+                       int sweep, hipStream_t stream);
+int fteik_k_statics(float* tstatic, const float* tobs, const float* tt,
+                    const float* wts, const float* t0, int ldo, int nobs,
+                    int nev, hipStream_t stream);
+// locate.hip (continued)
+int fteik_k_locate_t0_at(float* t0, const float* tables, int ntf, long ldgrd,
+                         const float* obs, const float* wt,
+                         const unsigned char* skip, const float* xnorm,
+                         int lntf, const int* cells, int nev, int ngrd,
+                         hipStream_t stream);
+}

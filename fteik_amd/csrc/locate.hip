@@ -1,0 +1,266 @@
+// locate.hip -- fused L2-misfit + argmin grid search on gfx950.
+//
+// locate.F90:620-744 (locate_computeL2ObjectiveFunction32f) followed by the
+// MINLOC of locate.F90:122, in one pass over the travel-time tables:
+//   pass 1  t0(c)  = sum_i w_i * (xnorm * (o_i - T_ic))          (:684-687)
+//   pass 2  obj(c) = sum_i ((sqrt(1/2) w_i) * (o_i - (T_ic + t0)))^2 (:694-705)
+// Each thread keeps its cell's station times in registers, so the tables
+// stream from HBM once (the reference reads them twice).  The static
+// corrections of the reference are identically zero (no setter exists,
+// locate.F90:210) and T + 0 == T, so they are not added.  The argmin packs
+// (objective bits << 32 | cell) into one 64-bit key: non-negative floats
+// order like their bit patterns, so the smallest key is the first minimum,
+// exactly MINLOC's tie-break.  Compiled without FMA contraction so every
+// objective value equals the reference's bit for bit.
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace {
+
+constexpr int TL = 256;
+constexpr int MAXT = 64;      // tables held in registers
+
+__device__ __forceinline__ unsigned long long make_key(float obj, int cell)
+{
+    return ((unsigned long long) __float_as_uint(obj) << 32)
+         | (unsigned long long) (unsigned) cell;
+}
+
+__device__ __forceinline__ unsigned long long wave_min(unsigned long long k)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(k, off, 64);
+        k = (o < k) ? o : k;
+    }
+    return k;
+}
+
+__global__ void k_tables_d2f(float* __restrict__ t, const double* __restrict__ d,
+                             int n)
+{
+    int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i < n) t[i] = (float) d[i];   // locate.F90:338 REAL(): round to nearest
+}
+
+// One event over all cells (locate_locateEvent).  t0work carries the
+// reference's state between events when accumulate != 0 (locate.F90:212,
+// :687: it is never reset between events).
+__global__ __launch_bounds__(TL)
+void k_locate_one(const float* __restrict__ tab, int ntf, int ngrd, long ldgrd,
+                  const float* __restrict__ obs, const float* __restrict__ wt,
+                  const unsigned char* __restrict__ skip, float xnorm,
+                  float* __restrict__ t0work, float* __restrict__ objwork,
+                  int accumulate, unsigned long long* __restrict__ best)
+{
+    __shared__ unsigned long long red[TL/64];
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    unsigned long long key = ~0ull;
+    if (c < ngrd) {
+        const float sqrtHalf = 0.70710677f;   // SQRT(0.5) in single precision
+        float T[MAXT];
+        float t0 = accumulate ? t0work[c] : 0.0f;
+        #pragma unroll
+        for (int i = 0; i < MAXT; i++) {
+            if (i < ntf) {
+                T[i] = tab[(long) i*ldgrd + c];
+                if (!skip[i]) {
+                    float res = xnorm*(obs[i] - T[i]);
+                    t0 = t0 + wt[i]*res;
+                }
+            }
+        }
+        for (int i = MAXT; i < ntf; i++) {
+            if (skip[i]) continue;
+            float res = xnorm*(obs[i] - tab[(long) i*ldgrd + c]);
+            t0 = t0 + wt[i]*res;
+        }
+        float obj = 0.0f;
+        #pragma unroll
+        for (int i = 0; i < MAXT; i++) {
+            if (i < ntf && !skip[i]) {
+                float w = sqrtHalf*wt[i];
+                float est = T[i] + t0;
+                float res = w*(obs[i] - est);
+                obj = obj + res*res;
+            }
+        }
+        for (int i = MAXT; i < ntf; i++) {
+            if (skip[i]) continue;
+            float w = sqrtHalf*wt[i];
+            float est = tab[(long) i*ldgrd + c] + t0;
+            float res = w*(obs[i] - est);
+            obj = obj + res*res;
+        }
+        if (accumulate) t0work[c] = t0;
+        if (objwork) objwork[c] = obj;
+        if (!accumulate && t0work) t0work[c] = t0;
+        key = make_key(obj, c);
+    }
+    key = wave_min(key);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long k = red[0];
+        for (int i = 1; i < TL/64; i++) k = (red[i] < k) ? red[i] : k;
+        atomicMin(best, k);
+    }
+}
+
+// Many events, fresh origin-time state per event (the analytic origin-time
+// removal of the grid search).  A thread holds CPT cells' tables in
+// registers and loops over the events; event data are uniform (scalar
+// loads).  best[e] receives the per-event key.
+template <int NTF, int CPT>
+__global__ __launch_bounds__(TL)
+void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
+                    const float* __restrict__ obs, const float* __restrict__ wt,
+                    const unsigned char* __restrict__ skip,
+                    const float* __restrict__ xnorm, int lntf, int nev,
+                    unsigned long long* __restrict__ best)
+{
+    __shared__ unsigned long long red[TL/64];
+    const float sqrtHalf = 0.70710677f;
+    const long base = ((long) blockIdx.x*TL*CPT) + threadIdx.x;
+    float T[CPT][NTF];
+    #pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const long c = base + (long) q*TL;
+        #pragma unroll
+        for (int i = 0; i < NTF; i++)
+            T[q][i] = (c < ngrd) ? tab[(long) i*ldgrd + c] : 0.0f;
+    }
+    for (int e = 0; e < nev; e++) {
+        const float* o = obs + (long) e*lntf;
+        const float* w = wt + (long) e*lntf;
+        const unsigned char* sk = skip + (long) e*lntf;
+        const float xn = xnorm[e];
+        float t0[CPT], ob[CPT];
+        #pragma unroll
+        for (int q = 0; q < CPT; q++) { t0[q] = 0.0f; ob[q] = 0.0f; }
+        #pragma unroll
+        for (int i = 0; i < NTF; i++) {
+            if (sk[i]) continue;
+            const float oi = o[i], wi = w[i];
+            #pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                float res = xn*(oi - T[q][i]);
+                t0[q] = t0[q] + wi*res;
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < NTF; i++) {
+            if (sk[i]) continue;
+            const float oi = o[i], wi = sqrtHalf*w[i];
+            #pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                float est = T[q][i] + t0[q];
+                float res = wi*(oi - est);
+                ob[q] = ob[q] + res*res;
+            }
+        }
+        unsigned long long key = ~0ull;
+        #pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const long c = base + (long) q*TL;
+            if (c < ngrd) {
+                unsigned long long k = make_key(ob[q], (int) c);
+                key = (k < key) ? k : key;
+            }
+        }
+        key = wave_min(key);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long k = red[0];
+            for (int i = 1; i < TL/64; i++) k = (red[i] < k) ? red[i] : k;
+            atomicMin(best + e, k);
+        }
+        __syncthreads();
+    }
+}
+
+// locate.F90:684-688 at one cell per event (origin time of the optimum).
+__global__ void k_t0_at(float* __restrict__ t0, const float* __restrict__ tab,
+                        int ntf, long ldgrd, const float* __restrict__ obs,
+                        const float* __restrict__ wt,
+                        const unsigned char* __restrict__ skip,
+                        const float* __restrict__ xnorm, int lntf,
+                        const int* __restrict__ cells, int nev, int ngrd)
+{
+    const int e = blockIdx.x*blockDim.x + threadIdx.x;
+    if (e >= nev) return;
+    const long c = cells[e];
+    if (c < 0 || c >= ngrd) { t0[e] = 0.0f; return; }
+    const float* o = obs + (long) e*lntf;
+    const float* w = wt + (long) e*lntf;
+    const unsigned char* sk = skip + (long) e*lntf;
+    float t = 0.0f;
+    for (int i = 0; i < ntf; i++) {
+        if (sk[i]) continue;
+        float res = xnorm[e]*(o[i] - tab[(long) i*ldgrd + c]);
+        t = t + w[i]*res;
+    }
+    t0[e] = t;
+}
+
+inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
+
+} // namespace
+
+extern "C" {
+
+int fteik_k_tables_d2f(float* tables, const double* tt, int ngrd,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tables_d2f, dim3((ngrd + 255)/256), dim3(256), 0, s,
+                       tables, tt, ngrd);
+    return ok();
+}
+
+int fteik_k_locate(const float* tables, int ntf, int ngrd, long ldgrd,
+                   const float* obs, const float* wt, const unsigned char* skip,
+                   int lntf, float xnorm, float* t0work, float* objwork,
+                   int accumulate, unsigned long long* best, hipStream_t s)
+{
+    (void) lntf;
+    hipLaunchKernelGGL(k_locate_one, dim3((ngrd + TL - 1)/TL), dim3(TL), 0, s,
+                       tables, ntf, ngrd, ldgrd, obs, wt, skip, xnorm, t0work,
+                       objwork, accumulate, best);
+    return ok();
+}
+
+int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
+                         const float* obs, const float* wt,
+                         const unsigned char* skip, const float* xnorm,
+                         int lntf, int nev, unsigned long long* best,
+                         float* t0best, hipStream_t s)
+{
+    (void) t0best;
+    constexpr int CPT = 2;
+    const int nblk = (int) ((ngrd + (long) TL*CPT - 1)/((long) TL*CPT));
+#define LB(N) case N: hipLaunchKernelGGL((k_locate_batch<N, CPT>), dim3(nblk), \
+        dim3(TL), 0, s, tables, ngrd, ldgrd, obs, wt, skip, xnorm, lntf, nev, \
+        best); break;
+    switch (ntf) {
+        LB(1) LB(2) LB(3) LB(4) LB(5) LB(6) LB(7) LB(8) LB(10) LB(12) LB(16)
+        LB(20) LB(24) LB(32) LB(40) LB(48) LB(56) LB(64)
+        default: return 2;
+    }
+#undef LB
+    return ok();
+}
+
+int fteik_k_locate_t0_at(float* t0, const float* tables, int ntf, long ldgrd,
+                         const float* obs, const float* wt,
+                         const unsigned char* skip, const float* xnorm,
+                         int lntf, const int* cells, int nev, int ngrd,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_t0_at, dim3((nev + 127)/128), dim3(128), 0, s, t0,
+                       tables, ntf, ldgrd, obs, wt, skip, xnorm, lntf, cells,
+                       nev, ngrd);
+    return ok();
+}
+
+} // extern "C"

@@ -1,0 +1,523 @@
+// sweep3d.hip -- 3D level-set fast sweeping on gfx950.
+//
+// One sweep of the reference (autoCode.f90:1001-1316 and siblings: every
+// masked node updated once, level by level, levels = flipped iz+ix+iy) runs
+// here as one kernel launch organised as a pipelined column wavefront:
+//
+//   * the (x, y) plane of the sweep's flipped frame is cut into blocks of
+//     16 x 16 columns; one 256-thread workgroup owns a block, one thread owns
+//     a column and walks down it, one node per step;
+//   * at step t every column processes the node on global level t, so all
+//     nodes a node depends on were produced at steps t-1..t-3; values of the
+//     last four steps live in an LDS ring, one barrier per step;
+//   * the upwind block neighbours (x-1, y-1, corner) publish a progress
+//     counter every H steps; a block pulls its halo columns for the next H
+//     steps with coherent loads once its producers are far enough ahead.
+//     Producers are always geometrically 16 steps ahead, so in steady state
+//     nobody waits and the whole sweep takes ~nx+ny+nz steps;
+//   * blocks are numbered upwind-first, so every dependency points to a
+//     lower workgroup id (in-order dispatch cannot deadlock); launches use
+//     cooperative launch whenever the grid fits, and every wait has a spin
+//     limit that turns a hang into a reported error.
+//
+// Kernel boundaries between sweeps give the coherence the next sweep needs.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "stencils.h"
+#include "kernels.h"
+
+namespace fteik {
+
+constexpr int BX = 16;           // columns per block along flipped x
+constexpr int BY = 16;           // columns per block along flipped y
+constexpr int NT = BX*BY;        // threads per workgroup
+constexpr int HB = 8;            // halo batch (steps per progress publish)
+constexpr int RING = 16;         // LDS ring depth (>= HB + 3)
+constexpr int HCOLS = BX + BY + 1;
+constexpr unsigned long long kSpinLimit = 1ull << 22;
+
+// ---------------------------------------------------------------- utils
+__device__ __forceinline__ void store_coherent(double* p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_coherent(float* p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_coherent(const double* p)
+{
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float load_coherent(const float* p)
+{
+    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Flipped 0-based coordinate -> actual 1-based index.
+__device__ __forceinline__ int unflip(int f, int n, int sgn)
+{
+    return (sgn > 0) ? f + 1 : n - f;
+}
+
+template <typename R>
+__global__ void fill_kernel(R* __restrict__ a, long n, R v)
+{
+    long i = (long) blockIdx.x*blockDim.x + threadIdx.x;
+    long stride = (long) gridDim.x*blockDim.x;
+    for (; i < n; i += stride) a[i] = v;
+}
+
+// localSolver.F90:878-956 + solver3D.F90:1201-1202: mesh constants, the
+// source-cell slowness and the 8 analytic seed times.  Single thread.
+template <typename R>
+__global__ void source3d_init_kernel(Consts<R>* __restrict__ cst,
+                                     R* __restrict__ tt,
+                                     const R* __restrict__ slow,
+                                     SourceGeom g)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Consts<R> c;
+    const R one = R(1);
+    c.dz = R(g.dz); c.dx = R(g.dx); c.dy = R(g.dy);
+    c.zsa = R(g.zsa); c.xsa = R(g.xsa); c.ysa = R(g.ysa);
+    c.zsi = g.zsi; c.xsi = g.xsi; c.ysi = g.ysi; c.pad = 0;
+    long icell = (long) (g.ysi - 1)*(g.nz - 1)*(g.nx - 1)
+               + (long) (g.xsi - 1)*(g.nz - 1) + g.zsi - 1;
+    c.szero = slow[icell];
+    c.szero2 = c.szero*c.szero;
+    R hmin = rmin(rmin(c.dz, c.dx), c.dy);
+    c.epsSolver = R(g.eps)*hmin*c.szero;
+    R dz2 = c.dz*c.dz, dx2 = c.dx*c.dx, dy2 = c.dy*c.dy;
+    c.dzi = one/c.dz; c.dxi = one/c.dx; c.dyi = one/c.dy;
+    c.dz2i = one/dz2; c.dx2i = one/dx2; c.dy2i = one/dy2;
+    R dsum = c.dz2i + c.dx2i + c.dy2i;
+    c.dsum_nine = dsum*R(9);
+    c.dsumi = one/dsum;
+    c.dz2dx2 = one/(dz2*dx2);
+    c.dz2dy2 = one/(dz2*dy2);
+    c.dx2dy2 = one/(dx2*dy2);
+    c.dz2i_dx2i = c.dz2i*c.dx2i;
+    c.dz2i_dy2i = c.dz2i*c.dy2i;
+    c.dx2i_dy2i = c.dx2i*c.dy2i;
+    c.dz2i_p_dy2i_inv = one/(c.dz2i + c.dy2i);
+    c.dz2i_p_dx2i_inv = one/(c.dz2i + c.dx2i);
+    c.dx2i_p_dy2i_inv = one/(c.dx2i + c.dy2i);
+    c.dz2i_p_dx2i = c.dz2i + c.dx2i;
+    c.dz2i_p_dy2i = c.dz2i + c.dy2i;
+    c.dx2i_p_dy2i = c.dx2i + c.dy2i;
+    *cst = c;
+    if (tt) {
+        for (int n = 0; n < 8; n++) {
+            int i = c.zsi + (n & 1), j = c.xsi + ((n >> 1) & 1),
+                k = c.ysi + ((n >> 2) & 1);
+            long g0 = (long) (k - 1)*g.nz*g.nx + (long) (j - 1)*g.nz + i - 1;
+            tt[g0] = tana3d(i, j, k, c);
+        }
+    }
+}
+
+// Derive single-precision constants from the double ones (mixed mode).
+__global__ void consts_to_float_kernel(const Consts<double>* __restrict__ cd,
+                                       Consts<float>* __restrict__ cf)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const Consts<double>& a = *cd;
+    Consts<float> b;
+#define CV(f) b.f = (float) a.f
+    CV(dz); CV(dx); CV(dy); CV(dzi); CV(dxi); CV(dyi); CV(dz2i); CV(dx2i);
+    CV(dy2i); CV(dsum_nine); CV(dsumi); CV(epsSolver); CV(dz2dx2); CV(dz2dy2);
+    CV(dx2dy2); CV(dz2i_dx2i); CV(dz2i_dy2i); CV(dx2i_dy2i); CV(dz2i_p_dx2i);
+    CV(dz2i_p_dy2i); CV(dx2i_p_dy2i); CV(dz2i_p_dy2i_inv); CV(dz2i_p_dx2i_inv);
+    CV(dx2i_p_dy2i_inv); CV(szero); CV(szero2); CV(zsa); CV(xsa); CV(ysa);
+#undef CV
+    b.zsi = a.zsi; b.xsi = a.xsi; b.ysi = a.ysi; b.pad = 0;
+    *cf = b;
+}
+
+// ------------------------------------------------------- pencil sweep
+template <typename R>
+struct SweepArgs {
+    R* tt;
+    const R* slow;
+    const Consts<R>* cst;
+    unsigned long long* flags;   // one per block, (serial << 32) | step+1
+    int* err;
+    unsigned long long serial;
+    int nz, nx, ny;
+    int sz, sx, sy;              // time signs of the sweep
+    int fz0, fz1, fx0, fx1, fy0, fy1;   // update box, flipped 0-based
+    int nbx, nby;                // blocks along flipped x, y
+    double* maxdelta;            // optional: max |t_new - t_old| (as bits)
+};
+
+template <typename R>
+__device__ __forceinline__ void slowness7(const R* __restrict__ slow, int nz,
+                                          int nx, int ny, int i, int j, int k,
+                                          int sz, int sx, int sy, Slw<R>& s)
+{
+    // autoCode.f90:407-445: all 19 cells come from the clamped 2x2x2 cell
+    // neighbourhood; i1 = i - sgnv is im for + sweeps and ip for - sweeps.
+    const int im = max(i - 1, 1), ip = min(i, nz - 1);
+    const int jm = max(j - 1, 1), jp = min(j, nx - 1);
+    const int km = max(k - 1, 1), kp = min(k, ny - 1);
+    const long sz1 = nz - 1, szx = (long) (nz - 1)*(nx - 1);
+#define SC(a, b, c) slow[(long) ((c) - 1)*szx + (long) ((b) - 1)*sz1 + (a) - 1]
+    const R c000 = SC(im, jm, km), c100 = SC(ip, jm, km);
+    const R c010 = SC(im, jp, km), c110 = SC(ip, jp, km);
+    const R c001 = SC(im, jm, kp), c101 = SC(ip, jm, kp);
+    const R c011 = SC(im, jp, kp), c111 = SC(ip, jp, kp);
+#undef SC
+    // select by the sweep's sgnv: index 0 -> (i-1 side), 1 -> (i side)
+    const bool zi = (sz < 0), xj = (sx < 0), yk = (sy < 0);
+    // cells in plane i1 over (j, k)
+    const R a00 = zi ? c100 : c000, a10 = zi ? c110 : c010;
+    const R a01 = zi ? c101 : c001, a11 = zi ? c111 : c011;
+    s.s1 = rmin(rmin(rmin(a00, a01), a10), a11);
+    // cells in plane j1 over (i, k)
+    const R b00 = xj ? c010 : c000, b10 = xj ? c110 : c100;
+    const R b01 = xj ? c011 : c001, b11 = xj ? c111 : c101;
+    s.s2 = rmin(rmin(rmin(b00, b10), b01), b11);
+    // cells in plane k1 over (i, j)
+    const R d00 = yk ? c001 : c000, d10 = yk ? c101 : c100;
+    const R d01 = yk ? c011 : c010, d11 = yk ? c111 : c110;
+    s.s3 = rmin(rmin(rmin(d00, d01), d10), d11);
+    // line (i1, j1) over k
+    const R e0 = zi ? (xj ? c110 : c100) : (xj ? c010 : c000);
+    const R e1 = zi ? (xj ? c111 : c101) : (xj ? c011 : c001);
+    s.s4 = rmin(e0, e1);
+    // line (i1, k1) over j
+    const R f0 = zi ? (yk ? c101 : c100) : (yk ? c001 : c000);
+    const R f1 = zi ? (yk ? c111 : c110) : (yk ? c011 : c010);
+    s.s5 = rmin(f0, f1);
+    // line (j1, k1) over i
+    const R g0 = xj ? (yk ? c011 : c010) : (yk ? c001 : c000);
+    const R g1 = xj ? (yk ? c111 : c110) : (yk ? c101 : c100);
+    s.s6 = rmin(g0, g1);
+    // cell (i1, j1, k1)
+    s.s7 = zi ? (xj ? (yk ? c111 : c110) : (yk ? c101 : c100))
+              : (xj ? (yk ? c011 : c010) : (yk ? c001 : c000));
+}
+
+template <typename R, bool INIT>
+__global__ __launch_bounds__(NT)
+void sweep3d_pencil_kernel(SweepArgs<R> a)
+{
+    __shared__ R ring[RING][(BX + 1)*(BY + 1)];
+    __shared__ int s_abort;
+    const int wg = blockIdx.x;
+    const int fbx = wg / a.nby, fby = wg - (wg / a.nby)*a.nby;
+    const int fj0 = fbx*BX, fk0 = fby*BY;
+    const int lj = threadIdx.x % BX, lk = threadIdx.x / BX;
+    const int fj = fj0 + lj, fk = fk0 + lk;
+    const int nz = a.nz, nx = a.nx, ny = a.ny;
+    // extended box: update box plus the one frozen layer its nodes read
+    const int ez0 = a.fz0 - 1, ex0 = a.fx0 - 1, ey0 = a.fy0 - 1;
+    const int jlo = max(fj0, ex0), jhi = min(min(fj0 + BX - 1, a.fx1), nx - 1);
+    const int klo = max(fk0, ey0), khi = min(min(fk0 + BY - 1, a.fy1), ny - 1);
+    unsigned long long* myflag = a.flags + wg;
+    const unsigned long long hi = a.serial << 32;
+    if (jlo > jhi || klo > khi) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(myflag, hi | 0xffffffffull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const int tlo = ez0 + jlo + klo;
+    const int thi = a.fz1 + jhi + khi;
+    const bool colE = (fj >= jlo && fj <= jhi && fk >= klo && fk <= khi);
+    const bool colB = colE && fj >= a.fx0 && fk >= a.fy0;
+    const int j = unflip(fj, nx, a.sx), k = unflip(fk, ny, a.sy);
+    const long colbase = (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
+    const Consts<R> c = *a.cst;
+    const int me = (lk + 1)*(BX + 1) + (lj + 1);
+    const int cW = me - 1, cS = me - (BX + 1), cSW = me - (BX + 2);
+    R dmax = R(0);
+    if (threadIdx.x == 0) {
+        s_abort = 0;
+        __hip_atomic_store(myflag, hi | (unsigned long long) tlo,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+
+    for (int t = tlo; t <= thi; t++) {
+        if (((t - tlo) % HB) == 0) {
+            // halo batch: steps t-3 .. t+HB-2 of the upwind columns (the
+            // three oldest are needed by the first batch only)
+            const int need = t + HB - 2;
+            if (threadIdx.x < 3) {
+                int pbx = fbx - (threadIdx.x != 1 ? 1 : 0);
+                int pby = fby - (threadIdx.x != 0 ? 1 : 0);
+                if (pbx >= 0 && pby >= 0) {
+                    const unsigned long long* pf = a.flags + pbx*a.nby + pby;
+                    const unsigned long long want =
+                        hi | (unsigned long long) (need + 1);
+                    unsigned long long spins = 0;
+                    while (__hip_atomic_load(pf, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) < want) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > kSpinLimit) {
+                            atomicExch(a.err, 1);
+                            s_abort = 1;
+                            break;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (s_abort) {
+                // release the blocks that wait on this one; the error flag
+                // marks the whole sweep as failed
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(myflag, hi | 0xffffffffull,
+                                       __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            for (int h = threadIdx.x; h < HCOLS*(HB + 2); h += NT) {
+                int hc = h / (HB + 2), d = h - (h / (HB + 2))*(HB + 2);
+                int s = t - 3 + d;
+                int hj, hk, slot;
+                if (hc < BY)            { hj = fj0 - 1; hk = fk0 + hc;
+                                          slot = (hc + 1)*(BX + 1); }
+                else if (hc < BY + BX)  { hj = fj0 + (hc - BY); hk = fk0 - 1;
+                                          slot = hc - BY + 1; }
+                else                    { hj = fj0 - 1; hk = fk0 - 1; slot = 0; }
+                int fi = s - hj - hk;
+                if (hj >= 0 && hk >= 0 && hj < nx && hk < ny && fi >= 0 &&
+                    fi < nz) {
+                    int i = unflip(fi, nz, a.sz);
+                    int jj = unflip(hj, nx, a.sx), kk = unflip(hk, ny, a.sy);
+                    long g = (long) (kk - 1)*nz*nx + (long) (jj - 1)*nz + i - 1;
+                    ring[s & (RING - 1)][slot] = load_coherent(a.tt + g);
+                }
+            }
+            __syncthreads();
+        }
+        const int fi = t - fj - fk;
+        if (colE && fi >= ez0 && fi <= a.fz1) {
+            const int i = unflip(fi, nz, a.sz);
+            R* p = a.tt + colbase + i - 1;
+            R v = *p;
+            if (colB && fi >= a.fz0) {
+                const int s1 = (t - 1) & (RING - 1), s2 = (t - 2) & (RING - 1),
+                          s3 = (t - 3) & (RING - 1);
+                Nbr<R> nb;
+                nb.tv   = ring[s1][me];
+                nb.te   = ring[s1][cW];
+                nb.tn   = ring[s1][cS];
+                nb.tev  = ring[s2][cW];
+                nb.ten  = ring[s2][cSW];
+                nb.tnv  = ring[s2][cS];
+                nb.tnve = ring[s3][cSW];
+                Slw<R> sl;
+                slowness7(a.slow, nz, nx, ny, i, j, k, a.sz, a.sx, a.sy, sl);
+                R vn;
+                if (INIT) vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
+                else      vn = noinit3d(c, nb, sl, v);
+                dmax = rmax(dmax, v - vn);
+                v = vn;
+                store_coherent(p, v);
+            }
+            ring[t & (RING - 1)][me] = v;
+        }
+        if (((t - tlo) % HB) == HB - 1 || t == thi) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(myflag, hi | (unsigned long long) (t + 1),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0)
+        __hip_atomic_store(myflag, hi | 0xffffffffull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (a.maxdelta && dmax > R(0)) {
+        // positive doubles order like their bit patterns
+        double dd = (double) dmax;
+        atomicMax((unsigned long long*) a.maxdelta,
+                  (unsigned long long) __double_as_longlong(dd));
+    }
+}
+
+// Simple reference schedule: one launch per level (used for cross-checks
+// and as a fallback for grids whose column blocks cannot be co-resident).
+template <typename R, bool INIT>
+__global__ __launch_bounds__(256)
+void sweep3d_level_kernel(SweepArgs<R> a, int L)
+{
+    const int lx = a.fx1 - a.fx0 + 1, ly = a.fy1 - a.fy0 + 1;
+    long idx = (long) blockIdx.x*blockDim.x + threadIdx.x;
+    if (idx >= (long) lx*ly) return;
+    const int dk = (int) (idx / lx), dj = (int) (idx - (long) dk*lx);
+    const int fj = a.fx0 + dj, fk = a.fy0 + dk;
+    const int fi = L - fj - fk;
+    if (fi < a.fz0 || fi > a.fz1) return;
+    const int nz = a.nz, nx = a.nx, ny = a.ny;
+    const int i = unflip(fi, nz, a.sz), j = unflip(fj, nx, a.sx),
+              k = unflip(fk, ny, a.sy);
+    const long nzx = (long) nz*nx;
+    auto G = [&](int ii, int jj, int kk) -> long {
+        return (long) (kk - 1)*nzx + (long) (jj - 1)*nz + ii - 1;
+    };
+    const Consts<R> c = *a.cst;
+    Nbr<R> nb;
+    nb.tv   = a.tt[G(i - a.sz, j, k)];
+    nb.te   = a.tt[G(i, j - a.sx, k)];
+    nb.tn   = a.tt[G(i, j, k - a.sy)];
+    nb.tev  = a.tt[G(i - a.sz, j - a.sx, k)];
+    nb.ten  = a.tt[G(i, j - a.sx, k - a.sy)];
+    nb.tnv  = a.tt[G(i - a.sz, j, k - a.sy)];
+    nb.tnve = a.tt[G(i - a.sz, j - a.sx, k - a.sy)];
+    Slw<R> sl;
+    slowness7(a.slow, nz, nx, ny, i, j, k, a.sz, a.sx, a.sy, sl);
+    R v = a.tt[G(i, j, k)];
+    R vn = INIT ? init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy)
+                : noinit3d(c, nb, sl, v);
+    a.tt[G(i, j, k)] = vn;
+    if (a.maxdelta && v - vn > R(0))
+        atomicMax((unsigned long long*) a.maxdelta,
+                  (unsigned long long) __double_as_longlong((double) (v - vn)));
+}
+
+// ------------------------------------------------------------ launchers
+static void sweep_signs(int sweep, int& sz, int& sx, int& sy)
+{
+    // prefetch.py:4-8
+    static const int tz[8] = { 1,  1,  1,  1, -1, -1, -1, -1};
+    static const int tx[8] = { 1, -1,  1, -1,  1, -1,  1, -1};
+    static const int ty[8] = { 1,  1, -1, -1,  1,  1, -1, -1};
+    sz = tz[sweep - 1]; sx = tx[sweep - 1]; sy = ty[sweep - 1];
+}
+
+// solver3D.F90:493-565 box, converted to the flipped 0-based frame.
+static void flipped_box(int sweep, bool linit, int nz, int nx, int ny,
+                        int zsi, int xsi, int ysi, int* b)
+{
+    int sz, sx, sy;
+    sweep_signs(sweep, sz, sx, sy);
+    auto axis = [&](int sgn, int n, int si, int* lo, int* hi) {
+        // actual 1-based limits, then flip
+        int a0, a1;
+        if (sgn > 0) { a0 = linit ? max(2, si) : 2; a1 = n; }
+        else         { a0 = 1; a1 = linit ? si + 1 : n - 1; }
+        if (sgn > 0) { *lo = a0 - 1; *hi = a1 - 1; }
+        else         { *lo = n - a1; *hi = n - a0; }
+    };
+    axis(sz, nz, zsi, &b[0], &b[1]);
+    axis(sx, nx, xsi, &b[2], &b[3]);
+    axis(sy, ny, ysi, &b[4], &b[5]);
+}
+
+template <typename R>
+static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
+                          hipStream_t stream)
+{
+    SweepArgs<R> a;
+    a.tt = (R*) L.tt; a.slow = (const R*) L.slow;
+    a.cst = (const Consts<R>*) L.cst;
+    a.flags = L.flags; a.err = L.err; a.serial = L.serial;
+    a.nz = L.nz; a.nx = L.nx; a.ny = L.ny;
+    sweep_signs(sweep, a.sz, a.sx, a.sy);
+    int b[6];
+    flipped_box(sweep, linit, L.nz, L.nx, L.ny, L.zsi, L.xsi, L.ysi, b);
+    a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
+    a.fy0 = b[4]; a.fy1 = b[5];
+    a.maxdelta = L.maxdelta;
+    if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) return 0;
+    if (L.schedule == 1) {
+        const int nlev = (a.fz1 + a.fx1 + a.fy1) - (a.fz0 + a.fx0 + a.fy0) + 1;
+        const long ncol = (long) (a.fx1 - a.fx0 + 1)*(a.fy1 - a.fy0 + 1);
+        const int nblk = (int) ((ncol + 255)/256);
+        for (int l = 0; l < nlev; l++) {
+            int Lv = a.fz0 + a.fx0 + a.fy0 + l;
+            if (linit)
+                hipLaunchKernelGGL((sweep3d_level_kernel<R, true>), dim3(nblk),
+                                   dim3(256), 0, stream, a, Lv);
+            else
+                hipLaunchKernelGGL((sweep3d_level_kernel<R, false>), dim3(nblk),
+                                   dim3(256), 0, stream, a, Lv);
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 1;
+    }
+    a.nbx = (L.nx + BX - 1)/BX;
+    a.nby = (L.ny + BY - 1)/BY;
+    const int nblk = a.nbx*a.nby;
+    if (nblk > L.max_flags) return 2;
+    void* args[] = {&a};
+    hipError_t e;
+    const void* fn = linit ? (const void*) sweep3d_pencil_kernel<R, true>
+                           : (const void*) sweep3d_pencil_kernel<R, false>;
+    if (L.cooperative)
+        e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(NT), args, 0,
+                                       stream);
+    else
+        e = hipLaunchKernel(fn, dim3(nblk), dim3(NT), args, 0, stream);
+    return e == hipSuccess ? 0 : 1;
+}
+
+} // namespace fteik
+
+using namespace fteik;
+
+extern "C" int fteik_k_sweep3d(const SweepLaunch* L, int sweep, int linit,
+                               hipStream_t stream)
+{
+    if (L->fp32) return launch_sweep_t<float>(*L, sweep, linit != 0, stream);
+    return launch_sweep_t<double>(*L, sweep, linit != 0, stream);
+}
+
+extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32)
+{
+    // cooperative launch needs every block resident at once
+    const int nblk = ((nx + BX - 1)/BX)*((ny + BY - 1)/BY);
+    int per_cu = 0, ncu = 0, dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const void* fns[2];
+    fns[0] = fp32 ? (const void*) sweep3d_pencil_kernel<float, false>
+                  : (const void*) sweep3d_pencil_kernel<double, false>;
+    fns[1] = fp32 ? (const void*) sweep3d_pencil_kernel<float, true>
+                  : (const void*) sweep3d_pencil_kernel<double, true>;
+    for (int v = 0; v < 2; v++) {
+        per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[v], NT,
+                                                         0) != hipSuccess)
+            return 0;
+        if ((long) per_cu*ncu < nblk) return 0;
+    }
+    return 1;
+}
+
+extern "C" int fteik_k_source3d_init(void* cst, void* tt, const void* slow,
+                                     const SourceGeom* g, long ngrd, int fp32,
+                                     hipStream_t stream)
+{
+    const int nb = (int) std::min<long>((ngrd + 255)/256, 4096);
+    if (fp32) {
+        if (tt) hipLaunchKernelGGL(fill_kernel<float>, dim3(nb), dim3(256), 0,
+                                   stream, (float*) tt, ngrd, (float) kHuge);
+        hipLaunchKernelGGL(source3d_init_kernel<float>, dim3(1), dim3(64), 0,
+                           stream, (Consts<float>*) cst, (float*) tt,
+                           (const float*) slow, *g);
+    } else {
+        if (tt) hipLaunchKernelGGL(fill_kernel<double>, dim3(nb), dim3(256), 0,
+                                   stream, (double*) tt, ngrd, kHuge);
+        hipLaunchKernelGGL(source3d_init_kernel<double>, dim3(1), dim3(64), 0,
+                           stream, (Consts<double>*) cst, (double*) tt,
+                           (const double*) slow, *g);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+extern "C" int fteik_k_consts_to_float(const void* cd, void* cf,
+                                       hipStream_t stream)
+{
+    hipLaunchKernelGGL(consts_to_float_kernel, dim3(1), dim3(64), 0, stream,
+                       (const Consts<double>*) cd, (Consts<float>*) cf);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}

@@ -1,0 +1,270 @@
+/*
+ * fteik_amd.h -- C ABI of the MI355X fteik library (libfteik_shared.so).
+ *
+ * Every entry point below has the name, argument list and error convention
+ * of the reference's BIND(C) export it replaces (cited file:line of the
+ * reference), so existing C, Fortran and ctypes callers link against this
+ * library unchanged:
+ *   - scalars by value, arrays by pointer, `int *ierr` last
+ *     (0 = success, 1 = failure with a message on stderr);
+ *   - sources, receivers, fields and event numbers are 1-based exactly
+ *     where the reference's are (isrc, itf, evnmbr, obs2tf, optIndx);
+ *   - inputs are copied into library-owned (GPU) state; outputs are copied
+ *     into caller buffers; state is one process-global solver of each kind.
+ * Entry points marked EXTENSION have no counterpart in the reference.
+ *
+ * Ordering constants (constants.F90:28-39).
+ */
+#ifndef FTEIK_AMD_H
+#define FTEIK_AMD_H
+#include <stdbool.h>
+
+#define FTEIK_NATURAL_ORDERING 0
+#define FTEIK_ZXY_ORDERING     0
+#define FTEIK_ZX_ORDERING      0
+#define FTEIK_XYZ_ORDERING     1
+#define FTEIK_XZ_ORDERING      1
+#define FTEIK_ZYX_ORDERING     2
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ============================ 3D solver ============================== */
+/* solver3D.F90:126-132 */
+void fteik_solver3d_initialize(int nz, int nx, int ny,
+                               double z0, double x0, double y0,
+                               double dz, double dx, double dy,
+                               int nsweep, double eps, double convTol,
+                               int verbose, int *ierr);
+/* solver3D.F90:575 */
+void fteik_solver3d_free(void);
+/* solver3D.F90:1138 */
+void fteik_solver3d_setVelocityModel64f(int ncell, const double *vel,
+                                        int *ierr);
+/* solver3D.F90:1083 */
+void fteik_solver3d_setNodalVelocityModel64f(int ng, int order,
+                                             const double *vel, int *ierr);
+/* solver3D.F90:1115 */
+void fteik_solver3d_setCellVelocityModel64f(int nc, int order,
+                                            const double *vel, int *ierr);
+/* solver3D.F90:717 */
+void fteik_solver3d_setSources64f(int nsrc, const double *zsrc,
+                                  const double *xsrc, const double *ysrc,
+                                  int *ierr);
+/* solver3D.F90:1045 */
+void fteik_solver3d_setReceivers64f(int nrec, const double *zrec,
+                                    const double *xrec, const double *yrec,
+                                    int *ierr);
+/* solver3D.F90:1156 */
+void fteik_solver3d_solveSourceLSM(int isrc, int *ierr);
+/* solver3D.F90:990 */
+void fteik_solver3d_getTravelTimes64f(int nrec, double *ttr, int *ierr);
+/* solver3D.F90:844 */
+void fteik_solver3d_getTravelTimeField64f(int ngrd, int order, double *tt,
+                                          int *ierr);
+/* solver3D.F90:915 */
+void fteik_solver3d_getTravelTimeFieldCell64f(int ncell, int order,
+                                              double *tt, int *ierr);
+/* solver3D.F90:737 */
+void fteik_solver3d_getNumberOfSources(int *nsrc, int *ierr);
+/* solver3D.F90:1024 */
+void fteik_solver3d_getNumberOfReceivers(int *nrec, int *ierr);
+/* solver3D.F90:683 */
+void fteik_solver3d_setNumberOfSweeps(int nsweep, int *ierr);
+/* solver3D.F90:640 */
+void fteik_solver3d_setSphereToCartEpsilon(double eps, int *ierr);
+/* solver3D.F90:191 */
+void fteik_solver3d_setVerbosity(int verbose);
+/* solver3D.F90:206 */
+void fteik_solver3d_setConvergenceTolerance(double convTol);
+/* solver3D.F90:756, :784 -- ray tracing is not part of this library:
+ * these return ierr = 1. */
+void fteik_solver3d_traceRaysToReceivers(int *ierr);
+void fteik_solver3d_traceRaysToPoints(int np, const double *zp,
+                                      const double *xp, const double *yp,
+                                      int *ierr);
+/* EXTENSION: 0 = FP64 sweeps (reference semantics, default);
+ * 1 = mixed: FP64 initialization sweeps, FP32 Gauss-Seidel iterations,
+ *     then FP64 correction iterations until the largest update of an
+ *     iteration is <= convTol seconds (at most 16). */
+void fteik_solver3d_setPrecision(int mode, int *ierr);
+/* EXTENSION: 0 = pipelined column-block sweeps (default);
+ * 1 = one kernel launch per level (diagnostic schedule). */
+void fteik_solver3d_setSchedule(int schedule, int *ierr);
+/* EXTENSION: device time of the last solve (ms, hipEvent), number of FP64
+ * correction iterations run (mixed mode) and node updates performed. */
+void fteik_solver3d_getLastSolveInfo(double *ms, int *corrIterations,
+                                     double *updates, int *ierr);
+
+/* ============================ 2D solver ============================== */
+/* solver2D.F90:124 */
+void fteik_solver2d_initialize(int nz, int nx, double z0, double x0,
+                               double dz, double dx, int nsweep, double eps,
+                               double convTol, int verbose, int *ierr);
+/* solver2D.F90:233 */
+void fteik_solver2d_free(void);
+/* solver2D.F90:1688, :1637, :1665 */
+void fteik_solver2d_setVelocityModel64f(int ncell, const double *vel,
+                                        int *ierr);
+void fteik_solver2d_setNodalVelocityModel64f(int ng, int order,
+                                             const double *vel, int *ierr);
+void fteik_solver2d_setCellVelocityModel64f(int nc, int order,
+                                            const double *vel, int *ierr);
+/* solver2D.F90:1554 */
+void fteik_solver2d_setSources64f(int nsrc, const double *zsrc,
+                                  const double *xsrc, int *ierr);
+/* solver2D.F90:1512 */
+void fteik_solver2d_setReceivers64f(int nrec, const double *zrec,
+                                    const double *xrec, int *ierr);
+/* solver2D.F90:782, :1042, :740 */
+void fteik_solver2d_solveSourceLSM(int isrc, int *ierr);
+void fteik_solver2d_solveSourceFSM(int isrc, int *ierr);
+void fteik_solver2d_solveLSM(int *ierr);
+/* solver2D.F90:1295, :1345 */
+void fteik_solver2d_getTravelTimeField64f(int ngrd, int order, double *tt,
+                                          int *ierr);
+void fteik_solver2d_getTravelTimeFieldCell64f(int ncell, int order,
+                                              double *tt, int *ierr);
+/* solver2D.F90:1398: ttr is [ldr x nsrc] */
+void fteik_solver2d_getTravelTimes64f(int ldr, double *ttr, int *ierr);
+/* solver2D.F90:268, :1485, :1593, :205, :221 */
+void fteik_solver2d_getNumberOfReceivers(int *nrec, int *ierr);
+void fteik_solver2d_setNumberOfSweeps(int nsweep, int *ierr);
+void fteik_solver2d_setSphereToCartEpsilon(double eps, int *ierr);
+void fteik_solver2d_setVerbosity(int verbose);
+void fteik_solver2d_setConvergenceTolerance(double convTol);
+/* solver2D.F90:283, :310 -- not part of this library: ierr = 1. */
+void fteik_solver2d_traceRaysToReceivers(int *ierr);
+void fteik_solver2d_traceRaysToPoints(int np, const double *xp,
+                                      const double *zp, int *ierr);
+
+/* ============================== Locator ============================== */
+/* locate.F90:142 */
+void locate_initialize(int nEvents, int ntf, int ngrd, int *ierr);
+/* locate.F90:233 */
+void locate_finalize(void);
+/* locate.F90:287 */
+void locate_setTravelTimeField64f(int ngrd, int itf, const double *tt,
+                                  int *ierr);
+/* locate.F90:361 (float, although fteik_fortran.h:208 says double) */
+void locate_setTravelTimeField32f(int ngrd, int itf, const float *tt,
+                                  int *ierr);
+/* locate.F90:431 */
+void locate_setObservation64f(int evnmbr, int nttimes, bool lhaveOT,
+                              double t0, const int *obs2tf,
+                              const double *pickTimes, const double *wts,
+                              int *ierr);
+/* fwrapper.c:42: 0-based evnmbr and obs2tf */
+void locate_setObservation64fC(const int evnmbr, const int nttimes,
+                               const bool lhaveOT, const double t0In,
+                               const int *obs2tf, const double *pickTimes,
+                               const double *wts, int *ierr);
+/* locate.F90:511, :541, :573 */
+void locate_setNumberOfEvents(int nEvents, int *ierr);
+void locate_setNumberOfTravelTimeFields(int ntf, int *ierr);
+void locate_setNumberOfGridPointsF(int ngrd, int *ierr);
+/* locate.F90:112: optIndx is 1-based */
+void locate_locateEvent(int evnmbr, int *optIndx, double *t0Opt,
+                        double *objOpt);
+/* locate.F90:748 (all arguments by reference) */
+void locate_computeStatics32fF(const int *ldo, const int *nobs,
+                               const int *nev, const float *tobs,
+                               const float *tt, const float *t0,
+                               const float *wts, float *tstatic);
+/* EXTENSION: 1 (default) keeps the reference's behaviour exactly: the
+ * origin-time work array carries over from one locateEvent call to the next
+ * (locate.F90:212, :687) and, for fewer than 512 grid points, the zero
+ * padding of the objective array takes part in the minimum search
+ * (locate.F90:122, :200-203).  0 gives every event a fresh origin-time
+ * estimate and searches the grid points only. */
+void locate_setReferenceCompatibility(int mode, int *ierr);
+/* EXTENSION: locates events evnmbr[0..n-1] (1-based) in one pass over the
+ * tables, each with a fresh origin-time estimate. */
+void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
+                         double *t0Opt, double *objOpt, int *ierr);
+
+/* =========================== Model helpers =========================== */
+/* model.F90:771 */
+int fteik_model_grid2indexF(int i, int j, int k, int nz, int nzx);
+/* model.F90:827 */
+int fteik_model_velGrid2indexF(int i, int j, int k, int nzm1, int nzm1_nxm1);
+/* model.F90:794 (uses the 3D solver's grid) */
+void fteik_model_index2gridF(int igrd, int *i, int *j, int *k, int *ierr);
+/* fwrapper.c:8: 0-based */
+int fteik_model_grid2index(const int iz, const int ix, const int iy,
+                           const int nz, const int nzx);
+
+/* ============================== Analytic ============================= */
+/* analytic.F90:100, :208, :194, :346, :313, :408, :174 */
+void fteik_analytic_initialize64f(int nz, int nx, int ny, double z0,
+                                  double x0, double y0, double dz, double dx,
+                                  double dy, int verbose, int *ierr);
+void fteik_analytic_free(void);
+void fteik_analytic_setVerbosity(int verbose);
+void fteik_analytic_setReceivers64f(int nrec, const double *zrec,
+                                    const double *xrec, const double *yrec,
+                                    int *ierr);
+void fteik_analytic_setConstantVelocity64f(double vconst, int *ierr);
+void fteik_analytic_setLinearVelocityGradient64f(double v0, double v1,
+                                                 int *ierr);
+void fteik_analytic_setSources64f(int nsrc, const double *zsrc,
+                                  const double *xsrc, const double *ysrc,
+                                  int *ierr);
+/* analytic.F90:538, :574, :686, :747, :795 */
+void fteik_analytic_solveSourceConstantVelocity64f(int isrc, int *ierr);
+void fteik_analytic_solveSourceLinearVelocityGradient64f(int isrc, int *ierr);
+void fteik_analytic_getTravelTimeField64f(int ngrd, int order, double *tt,
+                                          int *ierr);
+void fteik_analytic_getTravelTimesConstantVel64f(int ldr, double *trec,
+                                                 int *ierr);
+void fteik_analytic_getTravelTimesGradientVel64f(int ldr, double *trec,
+                                                 int *ierr);
+
+/* =============================== Graph =============================== */
+/* graph3d.F90:82, :122, :392, :471, :448, :432, :359.  Levels are computed
+ * in closed form on the GPU (level = flipped iz+ix+iy, each level sorted by
+ * linear index); the solver itself does not need them. */
+void fteik_graph3d_initialize(int nz, int nx, int ny, int verbose, int *ierr);
+void fteik_graph3d_makeLevelStructures(int *ierr);
+void fteik_graph3d_getIJKVF(int nwork, int sweep, int *ijkv, int *ierr);
+void fteik_graph3d_getLevelPointerF(int nwork, int *levelPtr, int *ierr);
+int fteik_graph3d_getNumberOfLevels(int *ierr);
+int fteik_graph3d_getMaxLevelSize(int *ierr);
+void fteik_graph3d_free(void);
+
+/* ========================== HDF5 archive I/O ========================= */
+/* modules.F90:845-977 / h5io.F90: HDF5 is not part of this library; the
+ * symbols exist so python/fteik3d.py's binding table loads.  All return
+ * -1 (failure). */
+int fteik_h5io_initializeF(const char *fileName);
+int fteik_h5io_finalizeF(void);
+int fteik_h5io_writeLevelSchedulesF(void);
+int fteik_h5io_writeTravelTimesF(const char *dataName);
+int fteik_h5io_writeVelocityModelF(const char *dataName);
+
+/* ============================ Diagnostics ============================ */
+/* EXTENSION: evaluates the device 3D local solvers on n nodes.  in holds
+ * n rows [tv te tn tev ten tnv tnve tt0 s1..s7], node n rows
+ * [sweep i j k] (1-based).  Mesh constants come from the source of the
+ * last solve (or fteik_amd_prepareSource3d). */
+void fteik_amd_evalLocalSolver3d(int n, const double *in, const int *node,
+                                 double *outNoInit, double *outInit,
+                                 int *ierr);
+/* EXTENSION: computes the solver constants for source isrc without
+ * solving (so the stencil diagnostic can run). */
+void fteik_amd_prepareSource3d(int isrc, int *ierr);
+/* EXTENSION: device time per sweep launch (ms) of nrep Gauss-Seidel
+ * iterations (8 sweeps each) on the field of the last solve, in FP64
+ * (precision 0) or FP32 (precision 1, after a mixed solve), timed with
+ * hipEvents on the library's stream.  Leaves the field as after the extra
+ * sweeps. */
+void fteik_amd_timeSweeps3d(int precision, int nrep, double *msPerLaunch,
+                            int *ierr);
+/* EXTENSION: library build string. */
+const char *fteik_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FTEIK_AMD_H */

@@ -1,0 +1,247 @@
+"""GPU parity of the 3D level-set solver against the oracle.
+
+Nodes of one level never read each other, so any schedule that respects
+the levels yields the reference's field bit for bit: the FP64 device path
+must equal the oracle exactly (the specification allows 1e-12 relative;
+we ask for equality).  All calls go through the C-ABI.
+"""
+import ctypes
+from ctypes import byref, c_int
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_helpers import GPU3D, grid9, hetero, lib, oracle3d, _d, _i
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(*a):
+    print("case:", *a, flush=True)
+
+
+# ------------------------------------------------------------ stencils
+def test_local_solvers_match_reference_fixtures(fixtures):
+    """Device noinit/init stencils on the reference's own 1000 node inputs
+    (localSolver.F90:384-751, fixtures from the reference build)."""
+    nz, nx, ny, h, zs, xs, ys, eps = fixtures["ls3d_grid"]
+    nz, nx, ny = int(nz), int(nx), int(ny)
+    g = GPU3D(nz, nx, ny, h, nsweep=2, eps=eps)
+    g.set_cells(fixtures["ls3d_vel"])
+    g.set_sources(zs, xs, ys)
+    L = lib()
+    ierr = c_int(1)
+    L.fteik_amd_prepareSource3d(1, byref(ierr))
+    assert ierr.value == 0
+    rows = np.ascontiguousarray(fixtures["ls3d_in"])
+    nodes = np.ascontiguousarray(fixtures["ls3d_node"], dtype=np.int32)
+    n = rows.shape[0]
+    out_n, out_i = np.zeros(n), np.zeros(n)
+    L.fteik_amd_evalLocalSolver3d(n, _d(rows), _i(nodes), _d(out_n),
+                                  _d(out_i), byref(ierr))
+    assert ierr.value == 0
+    np.testing.assert_array_equal(out_n, fixtures["ls3d_noinit"])
+    np.testing.assert_array_equal(out_i, fixtures["ls3d_init"])
+
+
+# ------------------------------------------------------------- solves
+CASES = [
+    # nz, nx, ny, h, (zs, xs, ys) in cells, nsweep, eps, seed
+    (23, 19, 17, 15.0, (7.7, 10.2, 5.5), 2, 4.0, 3),
+    (40, 37, 33, 12.0, (30.25, 3.5, 20.0), 2, 5.0, 4),     # >1 block per axis
+    (17, 50, 45, 10.0, (0.0, 47.2, 1.1), 1, 3.0, 5),       # z at the surface
+    (33, 33, 33, 10.0, (16.0, 16.0, 16.0), 0, 5.0, 6),     # init sweeps only
+    (31, 18, 47, 20.0, (12.6, 15.9, 44.3), 3, 2.0, 7),     # near far corner
+]
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_solve_matches_oracle(case, schedule):
+    nz, nx, ny, h, (zc, xc, yc), nsweep, eps, seed = CASES[case]
+    _announce("solve3d", CASES[case], "schedule", schedule)
+    vel = hetero(nz, nx, ny, h, seed)
+    zs, xs, ys = zc * h, xc * h, yc * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, nsweep, eps)
+    g = GPU3D(nz, nx, ny, h, nsweep=nsweep, eps=eps, schedule=schedule)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    assert g.solve(1) == 0
+    np.testing.assert_array_equal(g.field(), ref)
+    assert ref.max() < orc.HUGE
+
+
+def test_config1_linear_gradient_101():
+    """The 101^3 workload: h = 10 m, v(z) = 3000 + 0.5 z, source at the
+    centre, 20 seeded surface receivers, 2 iterations.  The field equals
+    the oracle's bit for bit, receivers too, and stays within first-order
+    accuracy of the closed form (analytic.F90:456-528)."""
+    n, h = 101, 10.0
+    _announce("config1", n)
+    zc = (np.arange(n - 1) + 0.5) * h
+    vel = np.tile(3000.0 + 0.5 * zc, (n - 1) * (n - 1))
+    zs = xs = ys = 50 * h
+    ref, slow, src = oracle3d(vel, n, n, n, h, zs, xs, ys, 2, 5.0)
+    rng = np.random.default_rng(101)
+    xr = rng.uniform(0.0, (n - 1) * h, 20)
+    yr = rng.uniform(0.0, (n - 1) * h, 20)
+    zr = np.zeros(20)
+    g = GPU3D(n, n, n, h, nsweep=2, eps=5.0)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    g.set_receivers(zr, xr, yr)
+    assert g.solve(1) == 0
+    tt = g.field()
+    np.testing.assert_array_equal(tt, ref)
+    ttr_ref = orc.receivers(ref, grid9(n, n, n, h), zr, xr, yr)
+    np.testing.assert_array_equal(g.receivers(), ttr_ref)
+    vbot = 3000.0 + 0.5 * (n - 1) * h
+    a = orc.xfast_to_natural(
+        orc.analytic_gradient(n, n, n, h, h, h, 0.0, 0.0, 0.0, zs, xs, ys,
+                              3000.0, vbot), n, n, n)
+    rel = np.abs(tt - a) / np.maximum(a, 1e-12)
+    assert np.median(rel) < 2e-3 and rel.max() < 1e-2
+
+
+def test_getters_orders_and_cells():
+    nz, nx, ny, h = 21, 18, 14, 25.0
+    vel = hetero(nz, nx, ny, h, 11)
+    zs, xs, ys = 4.4 * h, 9.1 * h, 6.6 * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, 2, 5.0)
+    g = GPU3D(nz, nx, ny, h)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    assert g.solve(1) == 0
+    for order in (1, 2):
+        np.testing.assert_array_equal(
+            g.field(order), orc.field_reorder3d(ref, nz, nx, ny, order))
+    for order in (0, 1, 2):
+        np.testing.assert_array_equal(
+            g.cells(order), orc.field_cell3d(ref, nz, nx, ny, order))
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_nodal_model_orders(order):
+    """Nodal velocities (model.F90:311-440, quirks kept) in each order."""
+    nz, nx, ny, h = 15, 13, 12, 20.0
+    rng = np.random.default_rng(order)
+    vn = 3000.0 + 1000.0 * rng.random(nz * nx * ny)
+    slow = orc.nodal_slowness(vn, nz, nx, ny, order)
+    zs, xs, ys = 5.5 * h, 6.2 * h, 3.3 * h
+    src = orc.source_locate(grid9(nz, nx, ny, h), zs, xs, ys, slow)
+    ref = orc.solve3d(slow, nz, nx, ny, h, h, h, src, 2, 5.0, schedule=1)
+    g = GPU3D(nz, nx, ny, h)
+    ierr = c_int(1)
+    v = np.ascontiguousarray(vn)
+    g.L.fteik_solver3d_setNodalVelocityModel64f(v.size, order, _d(v),
+                                                byref(ierr))
+    assert ierr.value == 0
+    g.set_sources(zs, xs, ys)
+    assert g.solve(1) == 0
+    np.testing.assert_array_equal(g.field(), ref)
+
+
+def test_multiple_sources_and_origin():
+    """Several sources, solved one after another on the same state, with a
+    non-zero grid origin (source.F90:136-190)."""
+    nz, nx, ny, h = 26, 22, 19, 30.0
+    origin = (100.0, -250.0, 40.0)
+    vel = hetero(nz, nx, ny, h, 21)
+    srcs = [(3.3, 5.0, 7.7), (20.2, 1.0, 0.4), (12.0, 12.0, 9.0)]
+    g = GPU3D(nz, nx, ny, h, origin=origin)
+    g.set_cells(vel)
+    z = [origin[0] + s[0] * h for s in srcs]
+    x = [origin[1] + s[1] * h for s in srcs]
+    y = [origin[2] + s[2] * h for s in srcs]
+    g.set_sources(z, x, y)
+    for k in range(len(srcs)):
+        ref, _, _ = oracle3d(vel, nz, nx, ny, h, z[k], x[k], y[k], 2, 5.0,
+                             origin=origin)
+        assert g.solve(k + 1) == 0
+        np.testing.assert_array_equal(g.field(), ref)
+
+
+def test_mixed_precision_converges_to_fp64():
+    """FP32 iterations + FP64 correction iterations to convergence give the
+    FP64 fixed point to <= 1e-6 relative (most nodes far closer)."""
+    nz, nx, ny, h = 48, 45, 41, 20.0
+    vel = hetero(nz, nx, ny, h, 31)
+    zs, xs, ys = 20.3 * h, 11.1 * h, 30.7 * h
+    g = GPU3D(nz, nx, ny, h, nsweep=8)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    assert g.solve(1) == 0
+    t64 = g.field()
+    g8 = GPU3D(nz, nx, ny, h, nsweep=2, precision=1, conv=0.0)
+    g8.set_cells(vel)
+    g8.set_sources(zs, xs, ys)
+    assert g8.solve(1) == 0
+    tm = g8.field()
+    _, corr, _ = g8.info()
+    assert 1 <= corr <= 16
+    rel = np.abs(tm - t64) / np.maximum(t64, 1e-12)
+    assert rel.max() <= 1e-6, rel.max()
+
+
+def test_error_paths():
+    L = lib()
+    ierr = c_int(0)
+    g = GPU3D(12, 11, 10, 10.0)
+    # no model yet
+    g.set_sources(50.0, 50.0, 50.0)
+    L.fteik_solver3d_solveSourceLSM(1, byref(ierr))
+    assert ierr.value == 1
+    g.set_cells(np.full(11 * 10 * 9, 3000.0))
+    # source number out of range
+    L.fteik_solver3d_solveSourceLSM(2, byref(ierr))
+    assert ierr.value == 1
+    # source in the last cell along z (the reference reads out of bounds)
+    g.set_sources(10.95 * 10.0, 50.0, 50.0)
+    L.fteik_solver3d_solveSourceLSM(1, byref(ierr))
+    assert ierr.value == 1
+    # field requested before any successful solve
+    tt = np.zeros(12 * 11 * 10)
+    L.fteik_solver3d_getTravelTimeField64f(tt.size, 0, _d(tt), byref(ierr))
+    assert ierr.value == 1
+    # negative velocity
+    bad = np.full(11 * 10 * 9, 3000.0)
+    bad[17] = -1.0
+    L.fteik_solver3d_setVelocityModel64f(bad.size, _d(bad), byref(ierr))
+    assert ierr.value == 1
+    # bad schedule / precision values
+    L.fteik_solver3d_setSchedule(7, byref(ierr))
+    assert ierr.value == 1
+    L.fteik_solver3d_setPrecision(3, byref(ierr))
+    assert ierr.value == 1
+    # source outside the grid
+    z = np.array([-5.0])
+    L.fteik_solver3d_setSources64f(1, _d(z), _d(z), _d(z), byref(ierr))
+    assert ierr.value == 1
+
+
+def test_graph_levels_match_oracle():
+    """graph3d.F90: level pointers and (i, j, k, node) lists per sweep."""
+    L = lib()
+    nz, nx, ny = 9, 7, 6
+    ierr = c_int(1)
+    L.fteik_graph3d_initialize(nz, nx, ny, 0, byref(ierr))
+    assert ierr.value == 0
+    L.fteik_graph3d_makeLevelStructures(byref(ierr))
+    assert ierr.value == 0
+    nlev = L.fteik_graph3d_getNumberOfLevels(byref(ierr))
+    assert nlev == nz + nx + ny - 2
+    ngrd = nz * nx * ny
+    for sweep in range(1, 9):
+        ptr_ref, ijkv_ref = orc.levels3d(sweep, nz, nx, ny)
+        ptr = np.zeros(nlev + 1, dtype=np.int32)
+        L.fteik_graph3d_getLevelPointerF(nlev + 1, _i(ptr), byref(ierr))
+        assert ierr.value == 0
+        np.testing.assert_array_equal(ptr, ptr_ref)
+        ijkv = np.zeros(4 * ngrd, dtype=np.int32)
+        L.fteik_graph3d_getIJKVF(4 * ngrd, sweep, _i(ijkv), byref(ierr))
+        assert ierr.value == 0
+        np.testing.assert_array_equal(ijkv, ijkv_ref)
+    mx = L.fteik_graph3d_getMaxLevelSize(byref(ierr))
+    assert mx == int(np.max(np.diff(ptr_ref)))
+    L.fteik_graph3d_free()
