@@ -3,8 +3,9 @@
 Workload: grid nz = nx = ny = 257, h = 20 m, 8 flat layers at seeded random
 depths with velocities 1500 -> 6500 m/s increasing with depth, times a
 Gaussian-smoothed (sigma = 5 cells) +-5 % random perturbation; one source
-at a seeded random position; 8 sweep directions x 2 iterations after the
-initialization sweeps (the reference's default nsweep = 2).
+at a seeded random position; the initialization sweeps, then as many
+8-sweep Gauss-Seidel iterations as the FP64 field needs to stop changing
+(found at start-up and reported as nsweep).
 
 One step = one complete single-source solve (fteik_solver3d_solveSourceLSM)
 with the model and source already resident on the GPU.  The FP64 path
@@ -14,7 +15,8 @@ each timed over K steps after W warm-up steps.
 
 Output: one JSON line.  value = node updates per second of the FP64 solve
 (updates as the reference counts them: initialization masks plus
-nsweep x 8 x (n-1)^3), in Gupdates/s.  The mixed path's rate on the same
+nsweep x 8 x (n-1)^3), in Gupdates/s.  The mixed path runs nsweep FP32
+iterations and FP64 iterations until one changes nothing.  The mixed path's rate on the same
 update count, its speed-up and its error against the FP64 field are
 reported beside it.  roofline: the FP64 Gauss-Seidel sweep kernel, timed
 with hipEvents on the library's stream; cpu_baseline: the oracle's OpenMP
@@ -39,21 +41,24 @@ BYTES_PER_UPDATE = {"f64": 24.0, "f32": 12.0}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep3d_257_traffic.json")
 
 
-def layered_model(n=N, h=H, seed=2025):
-    """Cell velocities (natural order, z fastest) and a source position."""
+def layered_model(n=N, h=H):
+    """Cell velocities (natural order, z fastest) and a source position:
+    7 interface depths sorted U(0, (n-1) h) (seed 2), layer velocities
+    1500 -> 6500 m/s from top to bottom, times 1 + 0.05 g with g a
+    Gaussian-smoothed (sigma = 5 cells) N(0,1) field normalised to
+    max|g| = 1 (seed 3); source uniform over the cells whose indices are
+    all <= n-3 (0-based, seed 4)."""
     from scipy.ndimage import gaussian_filter
-    rng = np.random.default_rng(seed)
     nc = n - 1
-    depth = nc * h
-    interfaces = np.sort(rng.uniform(0.05 * depth, 0.95 * depth, 7))
+    interfaces = np.sort(np.random.default_rng(2).uniform(0.0, nc * h, 7))
     vlayer = np.linspace(1500.0, 6500.0, 8)
     zc = (np.arange(nc) + 0.5) * h
     v1d = vlayer[np.searchsorted(interfaces, zc)]
-    g = gaussian_filter(rng.standard_normal((nc, nc, nc)), sigma=5.0,
-                        mode="reflect")
+    g = gaussian_filter(np.random.default_rng(3).standard_normal((nc, nc, nc)),
+                        sigma=5.0, mode="reflect")
     g /= np.abs(g).max()
     vel = v1d[None, None, :] * (1.0 + 0.05 * g)
-    src = rng.uniform(2.0 * h, (n - 3) * h, 3)       # (z, x, y)
+    src = np.random.default_rng(4).uniform(0.0, (n - 2) * h, 3)  # (z, x, y)
     return np.ascontiguousarray(vel.ravel()), src
 
 
@@ -81,9 +86,9 @@ def main():
     vel, (zs, xs, ys) = layered_model()
     ierr = c_int(1)
 
-    def init(precision):
-        L.fteik_solver3d_initialize(N, N, N, 0.0, 0.0, 0.0, H, H, H, 2, 5.0,
-                                    0.0, 0, byref(ierr))
+    def init(precision, nsweep):
+        L.fteik_solver3d_initialize(N, N, N, 0.0, 0.0, 0.0, H, H, H, nsweep,
+                                    5.0, 0.0, 0, byref(ierr))
         assert ierr.value == 0, "initialize"
         L.fteik_solver3d_setVelocityModel64f(vel.size, vel.ctypes.data_as(
             ctypes.POINTER(c_double)), byref(ierr))
@@ -123,14 +128,35 @@ def main():
         assert ierr.value == 0, "timeSweeps3d"
         return ms.value
 
+    def timed(k, w):
+        run(w)
+        hip_sync()
+        t0 = time.perf_counter()
+        run(k)
+        hip_sync()
+        return (time.perf_counter() - t0) / k
+
+    # ------------------------------------------- FP64 iterations to converge
+    # The reference runs a fixed number of iterations (no convergence test);
+    # the workload uses the smallest count after which one more iteration
+    # changes no node.
+    prev, nsweep = None, None
+    for ns in range(1, 13):
+        init(0, ns)
+        run(1)
+        cur = field()
+        if prev is not None and np.array_equal(cur, prev):
+            nsweep = ns - 1
+            break
+        prev = cur
+    if nsweep is None:
+        nsweep = 12
+    print("bench: FP64 converges after %d iterations" % nsweep,
+          file=sys.stderr, flush=True)
+
     # ---------------------------------------------------------- FP64 path
-    init(0)
-    run(args.warmup)
-    hip_sync()
-    t0 = time.perf_counter()
-    run(args.steps)
-    hip_sync()
-    t64 = (time.perf_counter() - t0) / args.steps
+    init(0, nsweep)
+    t64 = timed(args.steps, args.warmup)
     _, _, upd64 = info()
     tt64 = field()
     print("bench: fp64 solve %.3f ms" % (t64 * 1e3), file=sys.stderr,
@@ -138,18 +164,14 @@ def main():
     ms_launch64 = sweep_ms(0)
 
     # --------------------------------------------------------- mixed path
-    init(1)
-    run(args.warmup)
-    hip_sync()
-    t0 = time.perf_counter()
-    run(args.steps)
-    hip_sync()
-    tmix = (time.perf_counter() - t0) / args.steps
+    init(1, nsweep)
+    tmix = timed(args.steps, args.warmup)
     _, corr, updmix = info()
     ttm = field()
     rel = np.abs(ttm - tt64) / np.maximum(tt64, 1e-12)
-    print("bench: mixed solve %.3f ms, %d FP64 correction iterations"
-          % (tmix * 1e3, corr), file=sys.stderr, flush=True)
+    print("bench: mixed solve %.3f ms, %d FP64 correction iterations, max "
+          "rel err %.3g" % (tmix * 1e3, corr, rel.max()), file=sys.stderr,
+          flush=True)
     ms_launch32 = sweep_ms(1)
 
     units = float((N - 1) ** 3)          # updates per Gauss-Seidel launch
@@ -170,7 +192,8 @@ def main():
         src = orc.source_locate((N, N, N, H, H, H, 0.0, 0.0, 0.0), zs, xs, ys,
                                 slow)
         t0 = time.perf_counter()
-        ref = orc.solve3d(slow, N, N, N, H, H, H, src, 2, 5.0, schedule=1)
+        ref = orc.solve3d(slow, N, N, N, H, H, H, src, nsweep, 5.0,
+                          schedule=1)
         tcpu = time.perf_counter() - t0
         cpu = {"value": upd64 / tcpu / 1e9, "unit": "Gupdates/s",
                "cores": cores, "kind": "port",
@@ -189,18 +212,19 @@ def main():
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "3D single-source level-set sweeps, 257^3, "
                                "h=20 m, 8 layers 1500-6500 m/s +-5% smoothed "
-                               "perturbation, nsweep=2",
-                   "grid": [N, N, N], "h": H, "nsweep": 2, "eps": 5.0,
-                   "source_zxy": [zs, xs, ys]},
+                               "perturbation, FP64 iterations to "
+                               "convergence",
+                   "grid": [N, N, N], "h": H, "nsweep": nsweep, "eps": 5.0,
+                   "source_zxy": [zs, xs, ys],
+                   "updates_per_solve": upd64},
         "mixed": {"value": upd64 / tmix / 1e9, "unit": "Gupdates/s",
                   "ms_per_step": tmix * 1e3, "speedup_vs_fp64": t64 / tmix,
+                  "fp32_iterations": nsweep,
                   "fp64_correction_iterations": corr,
                   "updates_performed": updmix,
                   "max_rel_err_vs_fp64": float(rel.max()),
                   "median_rel_err_vs_fp64": float(np.median(rel)),
-                  "fp32_sweep_ms_per_launch": ms_launch32,
-                  "fp32_sweep_GBps": BYTES_PER_UPDATE["f32"] * units
-                  / (ms_launch32 * 1e-3) / 1e9},
+                  "fp32_sweep_ms_per_launch": ms_launch32},
         "roofline": {"kernel": "sweep3d_pencil_kernel<double,false>",
                      "bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,

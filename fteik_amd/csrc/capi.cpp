@@ -240,6 +240,8 @@ struct Solver3D {
     double z0 = 0, x0 = 0, y0 = 0, dz = 0, dx = 0, dy = 0, eps = 0,
            convTol = 0;
     int precision = 0, schedule = 0, cooperative = 1;
+    double mixedBias = 1.0/262144.0;     // 2^-18 relative promotion margin
+    int maxCorr = 16;
     std::vector<SrcInfo> src;
     std::vector<int> ri;
     std::vector<double> rd;
@@ -735,6 +737,21 @@ void fteik_solver3d_setSchedule(int schedule, int *ierr)
     g3.schedule = schedule;
 }
 
+void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
+                                       int *ierr)
+{
+    *ierr = 0;
+    if (!(bias >= 0.0 && bias < 1.0e-2) || maxCorrections < 1 ||
+        maxCorrections > 64) {
+        msg("fteik_solver3d_setMixedParameters: need 0 <= bias < 1e-2 and "
+            "1 <= maxCorrections <= 64");
+        *ierr = 1;
+        return;
+    }
+    g3.mixedBias = bias;
+    g3.maxCorr = maxCorrections;
+}
+
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corr, double *updates,
                                      int *ierr)
 {
@@ -828,9 +845,9 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
         // merge: the FP64 initialization values are upper bounds of the
         // converged field, so keep the smaller of the two
         if (ok) ok = fteik_k_min_merge_f(g3.d_tt, g3.d_tt32, ngrd,
-                                         g_dev.stream) == 0;
+                                         g3.mixedBias, g_dev.stream) == 0;
         L.maxdelta = g3.d_maxdelta;
-        const int maxCorr = 16;
+        const int maxCorr = g3.maxCorr;
         for (corr = 0; corr < maxCorr && ok; ) {
             hipMemsetAsync(g3.d_maxdelta, 0, sizeof(double), g_dev.stream);
             for (int sw = 1; sw <= 8 && ok; sw++) {

@@ -101,7 +101,7 @@ int fteik_k_eval3d(const double* in, const int* node, const void* cst,
 
 extern "C" {
 // util.hip (continued)
-int fteik_k_min_merge_f(double* tt, const float* tt32, long n,
+int fteik_k_min_merge_f(double* tt, const float* tt32, long n, double bias,
                         hipStream_t stream);
 int fteik_k_graph_ijkv(int* ijkv, const int* levelPtr, int nz, int nx, int ny,
                        int sweep, hipStream_t stream);

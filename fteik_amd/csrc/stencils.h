@@ -12,6 +12,7 @@
 // single-precision sweeps of the mixed-precision mode).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "hypot_borges.h"
 
 namespace fteik {
 
@@ -329,7 +330,7 @@ __device__ __forceinline__ double tana2d(int i, int j, const Consts<double>& c)
 {
     double dz = (double(i) - c.zsa)*c.dz;
     double dx = (double(j) - c.xsa)*c.dx;
-    return c.szero*hypot(dx, dz);
+    return c.szero*hypot_borges(dx, dz);
 }
 
 // localSolver.F90:74-112

@@ -201,6 +201,23 @@ __device__ __forceinline__ void slowness7(const R* __restrict__ slow, int nz,
               : (xj ? (yk ? c011 : c010) : (yk ? c001 : c000));
 }
 
+// The 7 min-reduced slownesses from the node's 2x2x2 cell neighbourhood
+// given as u (upwind, the i1/j1/k1 side of autoCode.f90:407-409) and d
+// (other side) along each axis: c[z][x][y], index 0 = u, 1 = d.
+template <typename R>
+__device__ __forceinline__ void slowness7_ud(const R (&zu)[4], const R (&zd)[4],
+                                             Slw<R>& s)
+{
+    // zu/zd hold (x,y) = (u,u), (u,d), (d,u), (d,d)
+    s.s1 = rmin(rmin(rmin(zu[0], zu[1]), zu[2]), zu[3]);       // plane i1
+    s.s2 = rmin(rmin(rmin(zu[0], zd[0]), zu[1]), zd[1]);       // plane j1
+    s.s3 = rmin(rmin(rmin(zu[0], zu[2]), zd[0]), zd[2]);       // plane k1
+    s.s4 = rmin(zu[0], zu[1]);                                 // line i1,j1
+    s.s5 = rmin(zu[0], zu[2]);                                 // line i1,k1
+    s.s6 = rmin(zu[0], zd[0]);                                 // line j1,k1
+    s.s7 = zu[0];                                              // cell
+}
+
 template <typename R, bool INIT>
 __global__ __launch_bounds__(NT)
 void sweep3d_pencil_kernel(SweepArgs<R> a)
@@ -229,24 +246,82 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
     const int thi = a.fz1 + jhi + khi;
     const bool colE = (fj >= jlo && fj <= jhi && fk >= klo && fk <= khi);
     const bool colB = colE && fj >= a.fx0 && fk >= a.fy0;
-    const int j = unflip(fj, nx, a.sx), k = unflip(fk, ny, a.sy);
-    const long colbase = (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
+    // Loads of threads outside the extended box use a clamped in-grid
+    // column; their values are never used.
+    const int cj = min(max(fj, jlo), jhi), ck = min(max(fk, klo), khi);
+    const int j = unflip(cj, nx, a.sx), k = unflip(ck, ny, a.sy);
+    R* const col = a.tt + (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
+    // the column's four cell columns (autoCode.f90:407-445 clamps)
+    const int jm = max(j - 1, 1), jp = min(j, nx - 1);
+    const int km = max(k - 1, 1), kp = min(k, ny - 1);
+    const int xu = (a.sx > 0) ? jm : jp, xd = (a.sx > 0) ? jp : jm;
+    const int yu = (a.sy > 0) ? km : kp, yd = (a.sy > 0) ? kp : km;
+    const long szx = (long) (nz - 1)*(nx - 1);
+    const R* const c_uu = a.slow + (long) (yu - 1)*szx + (long) (xu - 1)*(nz - 1);
+    const R* const c_ud = a.slow + (long) (yd - 1)*szx + (long) (xu - 1)*(nz - 1);
+    const R* const c_du = a.slow + (long) (yu - 1)*szx + (long) (xd - 1)*(nz - 1);
+    const R* const c_dd = a.slow + (long) (yd - 1)*szx + (long) (xd - 1)*(nz - 1);
+    const int fdep = cj + ck;           // node of step t is fi = t - fdep
     const Consts<R> c = *a.cst;
     const int me = (lk + 1)*(BX + 1) + (lj + 1);
     const int cW = me - 1, cS = me - (BX + 1), cSW = me - (BX + 2);
     R dmax = R(0);
+
+    // 0-based offset of flipped node fi in its column, clamped to the box
+    auto node_off = [&](int fi) -> int {
+        fi = min(max(fi, ez0), a.fz1);
+        return (a.sz > 0) ? fi : nz - 1 - fi;
+    };
+    // 0-based z offset of the cell layer on the upwind side of flipped
+    // node f (its i1 plane); the downwind layer of f is the upwind one of
+    // f + 1.  Clamped as max(i-1,1) / min(i,nz-1) clamp the cell index.
+    auto layer_off = [&](int f) -> int {
+        const int zc = (a.sz > 0) ? f : nz - f;          // 1-based cell
+        return min(max(zc, 1), nz - 1) - 1;
+    };
+
+    // Register double buffer: own old times and cell layers of the next
+    // HB steps are loaded one batch ahead, so no step waits on memory.
+    R tn[HB], cn[HB + 1][4];
+    auto prefetch = [&](int tb) {
+        const int f0 = tb - fdep;
+        #pragma unroll
+        for (int q = 0; q < HB; q++) tn[q] = col[node_off(f0 + q)];
+        #pragma unroll
+        for (int q = 0; q <= HB; q++) {
+            const int z = layer_off(f0 + q);
+            cn[q][0] = c_uu[z]; cn[q][1] = c_ud[z];
+            cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
+        }
+    };
+
     if (threadIdx.x == 0) {
         s_abort = 0;
         __hip_atomic_store(myflag, hi | (unsigned long long) tlo,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // The initialization sweeps (heavier stencil, once per solve) load each
+    // batch at its start instead, which keeps their register count low
+    // enough for two workgroups per CU.
+    if (!INIT) prefetch(tlo);
     __syncthreads();
 
-    for (int t = tlo; t <= thi; t++) {
-        if (((t - tlo) % HB) == 0) {
-            // halo batch: steps t-3 .. t+HB-2 of the upwind columns (the
-            // three oldest are needed by the first batch only)
-            const int need = t + HB - 2;
+    for (int tb = tlo; tb <= thi; tb += HB) {
+        if (INIT) prefetch(tb);
+        R tc[HB], cc[HB + 1][4];
+        #pragma unroll
+        for (int q = 0; q < HB; q++) tc[q] = tn[q];
+        #pragma unroll
+        for (int q = 0; q <= HB; q++) {
+            cc[q][0] = cn[q][0]; cc[q][1] = cn[q][1];
+            cc[q][2] = cn[q][2]; cc[q][3] = cn[q][3];
+        }
+        if (!INIT && tb + HB <= thi) prefetch(tb + HB);
+
+        // halo batch: steps tb-3 .. tb+HB-2 of the upwind columns (the three
+        // oldest are needed by the first batch only)
+        {
+            const int need = tb + HB - 2;
             if (threadIdx.x < 3) {
                 int pbx = fbx - (threadIdx.x != 1 ? 1 : 0);
                 int pby = fby - (threadIdx.x != 0 ? 1 : 0);
@@ -278,59 +353,70 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
             }
             for (int h = threadIdx.x; h < HCOLS*(HB + 2); h += NT) {
                 int hc = h / (HB + 2), d = h - (h / (HB + 2))*(HB + 2);
-                int s = t - 3 + d;
+                int st = tb - 3 + d;
                 int hj, hk, slot;
                 if (hc < BY)            { hj = fj0 - 1; hk = fk0 + hc;
                                           slot = (hc + 1)*(BX + 1); }
                 else if (hc < BY + BX)  { hj = fj0 + (hc - BY); hk = fk0 - 1;
                                           slot = hc - BY + 1; }
                 else                    { hj = fj0 - 1; hk = fk0 - 1; slot = 0; }
-                int fi = s - hj - hk;
+                int fi = st - hj - hk;
                 if (hj >= 0 && hk >= 0 && hj < nx && hk < ny && fi >= 0 &&
                     fi < nz) {
                     int i = unflip(fi, nz, a.sz);
                     int jj = unflip(hj, nx, a.sx), kk = unflip(hk, ny, a.sy);
                     long g = (long) (kk - 1)*nz*nx + (long) (jj - 1)*nz + i - 1;
-                    ring[s & (RING - 1)][slot] = load_coherent(a.tt + g);
+                    ring[st & (RING - 1)][slot] = load_coherent(a.tt + g);
                 }
             }
             __syncthreads();
         }
-        const int fi = t - fj - fk;
-        if (colE && fi >= ez0 && fi <= a.fz1) {
-            const int i = unflip(fi, nz, a.sz);
-            R* p = a.tt + colbase + i - 1;
-            R v = *p;
-            if (colB && fi >= a.fz0) {
-                const int s1 = (t - 1) & (RING - 1), s2 = (t - 2) & (RING - 1),
-                          s3 = (t - 3) & (RING - 1);
-                Nbr<R> nb;
-                nb.tv   = ring[s1][me];
-                nb.te   = ring[s1][cW];
-                nb.tn   = ring[s1][cS];
-                nb.tev  = ring[s2][cW];
-                nb.ten  = ring[s2][cSW];
-                nb.tnv  = ring[s2][cS];
-                nb.tnve = ring[s3][cSW];
-                Slw<R> sl;
-                slowness7(a.slow, nz, nx, ny, i, j, k, a.sz, a.sx, a.sy, sl);
-                R vn;
-                if (INIT) vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
-                else      vn = noinit3d(c, nb, sl, v);
-                dmax = rmax(dmax, v - vn);
-                v = vn;
-                store_coherent(p, v);
+
+        #pragma unroll
+        for (int q = 0; q < HB; q++) {
+            const int t = tb + q;
+            if (t > thi) break;                       // block-uniform
+            const int fi = t - fj - fk;
+            if (colE && fi >= ez0 && fi <= a.fz1) {
+                R v = tc[q];
+                if (colB && fi >= a.fz0) {
+                    const int s1 = (t - 1) & (RING - 1),
+                              s2 = (t - 2) & (RING - 1),
+                              s3 = (t - 3) & (RING - 1);
+                    Nbr<R> nb;
+                    nb.tv   = ring[s1][me];
+                    nb.te   = ring[s1][cW];
+                    nb.tn   = ring[s1][cS];
+                    nb.tev  = ring[s2][cW];
+                    nb.ten  = ring[s2][cSW];
+                    nb.tnv  = ring[s2][cS];
+                    nb.tnve = ring[s3][cSW];
+                    Slw<R> sl;
+                    slowness7_ud<R>(cc[q], cc[q + 1], sl);
+                    R vn;
+                    if (INIT) {
+                        const int i = unflip(fi, nz, a.sz);
+                        vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
+                    } else {
+                        vn = noinit3d(c, nb, sl, v);
+                    }
+                    dmax = rmax(dmax, v - vn);
+                    v = vn;
+                    store_coherent(col + node_off(fi), v);
+                }
+                ring[t & (RING - 1)][me] = v;
             }
-            ring[t & (RING - 1)][me] = v;
-        }
-        if (((t - tlo) % HB) == HB - 1 || t == thi) {
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            if (threadIdx.x == 0)
-                __hip_atomic_store(myflag, hi | (unsigned long long) (t + 1),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __syncthreads();
+            if (q == HB - 1 || t == thi) {
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(myflag,
+                                       hi | (unsigned long long) (t + 1),
+                                       __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __syncthreads();
+            }
         }
     }
     if (threadIdx.x == 0)

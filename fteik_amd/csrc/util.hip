@@ -296,11 +296,18 @@ __global__ void k_analytic_reorder(double* __restrict__ out,
 
 
 // Mixed precision: keep the smaller of the FP64 field and the FP32 one.
+// Promotes single-precision times, raised by the relative margin bias,
+// into the double field, keeping the smaller value.  With the margin the
+// promoted field stays an upper bound of the double-precision solution even
+// where single-precision rounding undershot it, so the double sweeps that
+// follow (which can only lower times) converge onto that solution.
 __global__ void k_min_merge_f(double* __restrict__ tt,
-                              const float* __restrict__ t32, long n)
+                              const float* __restrict__ t32, long n,
+                              double bias)
 {
     GRID_STRIDE(i, n) {
-        const double v = (double) t32[i];
+        double v = (double) t32[i];
+        v = v + v*bias;
         tt[i] = (v < tt[i]) ? v : tt[i];
     }
 }
@@ -470,10 +477,11 @@ int fteik_k_analytic_reorder(double* out, const double* a, int nz, int nx,
     return ok();
 }
 
-int fteik_k_min_merge_f(double* tt, const float* t32, long n, hipStream_t s)
+int fteik_k_min_merge_f(double* tt, const float* t32, long n, double bias,
+                        hipStream_t s)
 {
     hipLaunchKernelGGL(k_min_merge_f, dim3(nblocks(n)), dim3(TPB), 0, s, tt,
-                       t32, n);
+                       t32, n, bias);
     return ok();
 }
 

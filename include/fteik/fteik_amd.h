@@ -85,13 +85,21 @@ void fteik_solver3d_traceRaysToPoints(int np, const double *zp,
                                       const double *xp, const double *yp,
                                       int *ierr);
 /* EXTENSION: 0 = FP64 sweeps (reference semantics, default);
- * 1 = mixed: FP64 initialization sweeps, FP32 Gauss-Seidel iterations,
- *     then FP64 correction iterations until the largest update of an
- *     iteration is <= convTol seconds (at most 16). */
+ * 1 = mixed: FP64 initialization sweeps, nsweep FP32 Gauss-Seidel
+ *     iterations, promotion to FP64 with a small upward margin, then FP64
+ *     iterations until the largest update of an iteration is <= convTol
+ *     seconds (see fteik_solver3d_setMixedParameters). */
 void fteik_solver3d_setPrecision(int mode, int *ierr);
 /* EXTENSION: 0 = pipelined column-block sweeps (default);
  * 1 = one kernel launch per level (diagnostic schedule). */
 void fteik_solver3d_setSchedule(int schedule, int *ierr);
+/* EXTENSION: mixed-mode parameters: the single-precision field is raised
+ * by the relative margin bias (default 2^-18) when it is promoted, so it
+ * stays an upper bound of the double-precision solution and the double
+ * correction iterations (at most maxCorrections, default 16) converge onto
+ * it.  Reset to the defaults by fteik_solver3d_initialize. */
+void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
+                                       int *ierr);
 /* EXTENSION: device time of the last solve (ms, hipEvent), number of FP64
  * correction iterations run (mixed mode) and node updates performed. */
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corrIterations,

@@ -163,12 +163,14 @@ def test_multiple_sources_and_origin():
 
 
 def test_mixed_precision_converges_to_fp64():
-    """FP32 iterations + FP64 correction iterations to convergence give the
-    FP64 fixed point to <= 1e-6 relative (most nodes far closer)."""
+    """FP32 iterations, promotion with an upward margin, then FP64
+    iterations until nothing changes: the result is the converged FP64
+    field to <= 1e-6 relative (the tolerance the mixed mode is specified
+    to); the FP64 reference runs 16 iterations, far past convergence."""
     nz, nx, ny, h = 48, 45, 41, 20.0
     vel = hetero(nz, nx, ny, h, 31)
     zs, xs, ys = 20.3 * h, 11.1 * h, 30.7 * h
-    g = GPU3D(nz, nx, ny, h, nsweep=8)
+    g = GPU3D(nz, nx, ny, h, nsweep=16)
     g.set_cells(vel)
     g.set_sources(zs, xs, ys)
     assert g.solve(1) == 0
@@ -179,8 +181,9 @@ def test_mixed_precision_converges_to_fp64():
     assert g8.solve(1) == 0
     tm = g8.field()
     _, corr, _ = g8.info()
-    assert 1 <= corr <= 16
+    assert 1 <= corr < 16          # converged before the cap
     rel = np.abs(tm - t64) / np.maximum(t64, 1e-12)
+    print("mixed: %d FP64 iterations, max rel err %.3g" % (corr, rel.max()))
     assert rel.max() <= 1e-6, rel.max()
 
 
