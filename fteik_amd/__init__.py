@@ -82,6 +82,8 @@ def load_library(path=None):
         "fteik_solver3d_setPrecision": [i, _ip],
         "fteik_solver3d_setSchedule": [i, _ip],
         "fteik_solver3d_setMixedParameters": [d, i, _ip],
+        "fteik_solver3d_setTile": [i, i, _ip],
+        "fteik_solver3d_getTile": [_ip, _ip, _ip, _ip],
         "fteik_solver3d_getLastSolveInfo": [_dp, _ip, _dp, _ip],
         "fteik_solver3d_free": [],
         "fteik_solver2d_initialize": [i, i, d, d, d, d, i, d, d, i, _ip],

@@ -240,6 +240,8 @@ struct Solver3D {
     double z0 = 0, x0 = 0, y0 = 0, dz = 0, dx = 0, dy = 0, eps = 0,
            convTol = 0;
     int precision = 0, schedule = 0, cooperative = 1;
+    int tile = 0, tile32 = 0;            // pipelined-sweep tile shapes
+    std::vector<char> tileOK;            // tile fits co-resident (both types)
     double mixedBias = 1.0/262144.0;     // 2^-18 relative promotion margin
     int maxCorr = 16;
     std::vector<SrcInfo> src;
@@ -303,6 +305,7 @@ SweepLaunch launch3d(bool fp32)
     L.fp32 = fp32 ? 1 : 0;
     L.schedule = g3.schedule;
     L.cooperative = g3.cooperative;
+    L.tile = fp32 ? g3.tile32 : g3.tile;
     return L;
 }
 
@@ -529,7 +532,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     if (!device_ok("fteik_solver3d_initialize")) { *ierr = 1; return; }
     const long ngrd = (long) nz*nx*ny;
     const long ncell = (long) (nz - 1)*(nx - 1)*(ny - 1);
-    g3.max_flags = ((nx + 15)/16)*((ny + 15)/16);
+    g3.max_flags = fteik_k_max_flags(nx, ny);
     bool ok = dalloc(&g3.d_slow, ncell) && dalloc(&g3.d_tt, ngrd) &&
               dalloc(&g3.d_scr, ngrd) &&
               dalloc((char **) &g3.d_cst, 512) &&
@@ -547,8 +550,18 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     hipMemsetAsync(g3.d_err, 0, sizeof(int), g_dev.stream);
     hipEventCreate(&g3.ev0);
     hipEventCreate(&g3.ev1);
-    g3.cooperative = fteik_k_pencil_fits(nx, ny, 0) &&
-                     fteik_k_pencil_fits(nx, ny, 1);
+    // tile shapes whose whole grid of workgroups can be co-resident (FP64
+    // and FP32 kernels); the first such one is the default
+    const int ntile = fteik_k_num_tiles();
+    g3.tileOK.assign((size_t) ntile, 0);
+    int first = -1;
+    for (int t = 0; t < ntile; t++) {
+        g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
+                                        fteik_k_pencil_fits(nx, ny, 1, t));
+        if (g3.tileOK[(size_t) t] && first < 0) first = t;
+    }
+    g3.cooperative = first >= 0;
+    g3.tile = g3.tile32 = first >= 0 ? first : 0;
     if (!g3.cooperative) g3.schedule = 1;
     if (!sync_ok()) { msg("fteik_solver3d_initialize: GPU error"); *ierr = 1;
                       return; }
@@ -750,6 +763,29 @@ void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
     }
     g3.mixedBias = bias;
     g3.maxCorr = maxCorrections;
+}
+
+void fteik_solver3d_setTile(int tile, int tile32, int *ierr)
+{
+    *ierr = 0;
+    const int n = (int) g3.tileOK.size();
+    if (!g3.init || tile < 0 || tile >= n || tile32 < 0 || tile32 >= n ||
+        !g3.tileOK[(size_t) tile] || !g3.tileOK[(size_t) tile32]) {
+        msg("fteik_solver3d_setTile: tile shapes (%d, %d) are not available "
+            "for this grid", tile, tile32);
+        *ierr = 1;
+        return;
+    }
+    g3.tile = tile;
+    g3.tile32 = tile32;
+}
+
+void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr)
+{
+    *ierr = g3.init ? 0 : 1;
+    *tile = g3.tile;
+    *tile32 = g3.tile32;
+    *ntiles = (int) g3.tileOK.size();
 }
 
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corr, double *updates,

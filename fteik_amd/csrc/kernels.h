@@ -27,6 +27,7 @@ struct SweepLaunch {
     int fp32;                    // 0: double, 1: float
     int schedule;                // 0: pipelined column blocks, 1: per level
     int cooperative;
+    int tile;                    // tile shape index of the pipelined sweep
 };
 
 } // namespace fteik
@@ -35,7 +36,9 @@ extern "C" {
 // sweep3d.hip
 int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
                     hipStream_t stream);
-int fteik_k_pencil_fits(int nx, int ny, int fp32);
+int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile);
+int fteik_k_num_tiles(void);
+int fteik_k_max_flags(int nx, int ny);
 int fteik_k_source3d_init(void* cst, void* tt, const void* slow,
                           const fteik::SourceGeom* g, long ngrd, int fp32,
                           hipStream_t stream);

@@ -28,12 +28,15 @@
 
 namespace fteik {
 
-constexpr int BX = 16;           // columns per block along flipped x
-constexpr int BY = 16;           // columns per block along flipped y
-constexpr int NT = BX*BY;        // threads per workgroup
-constexpr int HB = 8;            // halo batch (steps per progress publish)
-constexpr int RING = 16;         // LDS ring depth (>= HB + 3)
-constexpr int HCOLS = BX + BY + 1;
+// Tile shapes of the pipelined sweep: BX x BY columns per workgroup, halo
+// exchanged with the upwind workgroups every HB steps.  The autotuner picks
+// one per grid (fteik_solver3d_autotune); kTiles[0] is the default.
+struct TileShape { int bx, by, hb; };
+constexpr TileShape kTiles[] = {
+    {16, 16, 8}, {16, 16, 4}, {16, 16, 2}, {8, 8, 4}, {8, 8, 2}, {32, 8, 4},
+    {16, 8, 4},
+};
+constexpr int kNumTiles = (int) (sizeof(kTiles)/sizeof(kTiles[0]));
 constexpr unsigned long long kSpinLimit = 1ull << 22;
 
 // ---------------------------------------------------------------- utils
@@ -218,10 +221,14 @@ __device__ __forceinline__ void slowness7_ud(const R (&zu)[4], const R (&zd)[4],
     s.s7 = zu[0];                                              // cell
 }
 
-template <typename R, bool INIT>
-__global__ __launch_bounds__(NT)
+template <typename R, bool INIT, int BX, int BY, int HB>
+__global__ __launch_bounds__(BX*BY)
 void sweep3d_pencil_kernel(SweepArgs<R> a)
 {
+    constexpr int NT = BX*BY;
+    constexpr int RING = (HB + 3 <= 8) ? 8 : 16;     // LDS ring depth
+    constexpr int HCOLS = BX + BY + 1;
+    static_assert(HB + 3 <= RING, "ring too shallow");
     __shared__ R ring[RING][(BX + 1)*(BY + 1)];
     __shared__ int s_abort;
     const int wg = blockIdx.x;
@@ -499,6 +506,25 @@ static void flipped_box(int sweep, bool linit, int nz, int nx, int ny,
     axis(sy, ny, ysi, &b[4], &b[5]);
 }
 
+template <typename R, bool INIT>
+static const void* pencil_fn_t(int tile)
+{
+#define PF(n) case n: return (const void*) sweep3d_pencil_kernel<R, INIT, \
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
+    switch (tile) {
+        PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
+        default: return nullptr;
+    }
+#undef PF
+}
+static_assert(kNumTiles == 7, "pencil_fn_t lists every tile");
+
+template <typename R>
+static const void* pencil_fn(bool linit, int tile)
+{
+    return linit ? pencil_fn_t<R, true>(tile) : pencil_fn_t<R, false>(tile);
+}
+
 template <typename R>
 static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
                           hipStream_t stream)
@@ -530,19 +556,21 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
         }
         return hipGetLastError() == hipSuccess ? 0 : 1;
     }
-    a.nbx = (L.nx + BX - 1)/BX;
-    a.nby = (L.ny + BY - 1)/BY;
+    const int tile = (L.tile >= 0 && L.tile < kNumTiles) ? L.tile : 0;
+    const TileShape ts = kTiles[tile];
+    a.nbx = (L.nx + ts.bx - 1)/ts.bx;
+    a.nby = (L.ny + ts.by - 1)/ts.by;
     const int nblk = a.nbx*a.nby;
     if (nblk > L.max_flags) return 2;
     void* args[] = {&a};
     hipError_t e;
-    const void* fn = linit ? (const void*) sweep3d_pencil_kernel<R, true>
-                           : (const void*) sweep3d_pencil_kernel<R, false>;
+    const void* fn = pencil_fn<R>(linit, tile);
     if (L.cooperative)
-        e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(NT), args, 0,
-                                       stream);
+        e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(ts.bx*ts.by), args,
+                                       0, stream);
     else
-        e = hipLaunchKernel(fn, dim3(nblk), dim3(NT), args, 0, stream);
+        e = hipLaunchKernel(fn, dim3(nblk), dim3(ts.bx*ts.by), args, 0,
+                            stream);
     return e == hipSuccess ? 0 : 1;
 }
 
@@ -557,26 +585,40 @@ extern "C" int fteik_k_sweep3d(const SweepLaunch* L, int sweep, int linit,
     return launch_sweep_t<double>(*L, sweep, linit != 0, stream);
 }
 
-extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32)
+extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
 {
-    // cooperative launch needs every block resident at once
-    const int nblk = ((nx + BX - 1)/BX)*((ny + BY - 1)/BY);
-    int per_cu = 0, ncu = 0, dev = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const void* fns[2];
-    fns[0] = fp32 ? (const void*) sweep3d_pencil_kernel<float, false>
-                  : (const void*) sweep3d_pencil_kernel<double, false>;
-    fns[1] = fp32 ? (const void*) sweep3d_pencil_kernel<float, true>
-                  : (const void*) sweep3d_pencil_kernel<double, true>;
+    // cooperative launch needs every block of the grid resident at once
+    if (tile < 0 || tile >= kNumTiles) return 0;
+    const TileShape ts = kTiles[tile];
+    const int nblk = ((nx + ts.bx - 1)/ts.bx)*((ny + ts.by - 1)/ts.by);
+    int ncu = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) return 0;
     for (int v = 0; v < 2; v++) {
-        per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fns[v], NT,
-                                                         0) != hipSuccess)
-            return 0;
+        const void* fn = fp32 ? pencil_fn<float>(v == 1, tile)
+                              : pencil_fn<double>(v == 1, tile);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
+                                                         ts.bx*ts.by, 0)
+            != hipSuccess) return 0;
         if ((long) per_cu*ncu < nblk) return 0;
     }
     return 1;
+}
+
+extern "C" int fteik_k_num_tiles(void) { return kNumTiles; }
+
+extern "C" int fteik_k_max_flags(int nx, int ny)
+{
+    // flag slots for the smallest workgroup shape
+    int m = 0;
+    for (int t = 0; t < kNumTiles; t++) {
+        const int n = ((nx + kTiles[t].bx - 1)/kTiles[t].bx)
+                    * ((ny + kTiles[t].by - 1)/kTiles[t].by);
+        m = (n > m) ? n : m;
+    }
+    return m;
 }
 
 extern "C" int fteik_k_source3d_init(void* cst, void* tt, const void* slow,

@@ -100,6 +100,12 @@ void fteik_solver3d_setSchedule(int schedule, int *ierr);
  * it.  Reset to the defaults by fteik_solver3d_initialize. */
 void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
                                        int *ierr);
+/* EXTENSION: tile shape of the pipelined FP64 and FP32 sweeps (index into
+ * the library's table of workgroup shapes and halo batch sizes; see
+ * fteik_solver3d_autotune).  Only shapes whose workgroups can all be
+ * resident for this grid are accepted. */
+void fteik_solver3d_setTile(int tile, int tile32, int *ierr);
+void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
 /* EXTENSION: device time of the last solve (ms, hipEvent), number of FP64
  * correction iterations run (mixed mode) and node updates performed. */
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corrIterations,
