@@ -62,6 +62,166 @@ def layered_model(n=N, h=H):
     return np.ascontiguousarray(vel.ravel()), src
 
 
+def batch_models(ngrids=64, h=50.0):
+    """64 grids, each axis U{64..192} (seed 5); v = 2000 + U(0.3,0.8) z
+    (seed 6) times 1 + 0.03 g, g smoothed as for the 257^3 model (seed 7);
+    one source per grid at a random node with every index <= n-3 (seed 8)."""
+    from scipy.ndimage import gaussian_filter
+    dims = np.random.default_rng(5).integers(64, 193, (ngrids, 3))
+    grad = np.random.default_rng(6).uniform(0.3, 0.8, ngrids)
+    r7 = np.random.default_rng(7)
+    r8 = np.random.default_rng(8)
+    out = []
+    for k in range(ngrids):
+        nz, nx, ny = (int(v) for v in dims[k])
+        zc = (np.arange(nz - 1) + 0.5) * h
+        g = gaussian_filter(r7.standard_normal((ny - 1, nx - 1, nz - 1)),
+                            sigma=5.0, mode="reflect")
+        g /= np.abs(g).max()
+        vel = (2000.0 + grad[k] * zc)[None, None, :] * (1.0 + 0.03 * g)
+        node = [int(r8.integers(0, n - 2)) for n in (nz, nx, ny)]
+        out.append(((nz, nx, ny), np.ascontiguousarray(vel.ravel()),
+                    tuple(i * h for i in node)))
+    return out
+
+
+def run_batch(L, models, h, nsweep, tile):
+    ierr = c_int(1)
+    dp = ctypes.POINTER(c_double)
+    L.fteik_batch3d_initialize(len(models), byref(ierr))
+    assert ierr.value == 0
+    for k, ((nz, nx, ny), vel, (zs, xs, ys)) in enumerate(models):
+        L.fteik_batch3d_setGrid(k + 1, nz, nx, ny, 0.0, 0.0, 0.0, h, h, h,
+                                vel.ctypes.data_as(dp), zs, xs, ys,
+                                byref(ierr))
+        assert ierr.value == 0, "batch grid %d" % k
+    L.fteik_batch3d_setTile(tile, byref(ierr))
+    t0 = time.perf_counter()
+    L.fteik_batch3d_solve(nsweep, 5.0, byref(ierr))
+    wall = time.perf_counter() - t0
+    assert ierr.value == 0, "batch solve"
+    ms, upd, corr = c_double(), c_double(), c_int()
+    L.fteik_batch3d_getLastSolveInfo(byref(ms), byref(corr), byref(upd),
+                                     byref(ierr))
+    return ms.value, upd.value, wall
+
+
+def batch_section(L, ntiles):
+    """Config 3: 64 independent grids of uneven size swept together; the
+    tile shape is picked by timing every shape on the batch (autotuner)."""
+    h = 50.0
+    models = batch_models()
+    times = {}
+    for tile in range(ntiles):
+        ms, upd, _ = run_batch(L, models, h, 2, tile)
+        times[tile] = ms
+    best = min(times, key=times.get)
+    ms, upd, wall = run_batch(L, models, h, 2, best)
+    L.fteik_batch3d_free()
+    return {"workload": "64 grids, each axis U{64..192}, h=50 m, "
+                        "v=2000+U(0.3,0.8)z, +-3% smoothed, one source each, "
+                        "FP64, nsweep=2",
+            "grids": len(models), "updates": upd, "ms": ms,
+            "value": upd / (ms * 1e-3) / 1e9, "unit": "Gupdates/s",
+            "autotuned_tile": best,
+            "ms_by_tile": {str(k): v for k, v in times.items()}}
+
+
+def locator_workload():
+    """Config 4: 161 x 161 x 81 (nx = ny = 161, nz = 81), h = 250 m,
+    v = 4000 + 0.4 z; 48 stations at z = 0, x, y ~ U[0, 158 h) (seed 9);
+    10,000 events at uniform random nodes (seed 10), origin times U(0, 100) s
+    (seed 11), picks = T + origin + N(0, 0.05^2) (seed 12), weights 1."""
+    nz, nx, ny, h = 81, 161, 161, 250.0
+    zc = (np.arange(nz - 1) + 0.5) * h
+    vel = np.ascontiguousarray(np.tile(4000.0 + 0.4 * zc,
+                                       (nx - 1) * (ny - 1)))
+    st = np.random.default_rng(9).uniform(0.0, 158.0 * h, (48, 2))
+    stations = [(0.0, float(x), float(y)) for x, y in st]
+    return (nz, nx, ny, h), vel, stations
+
+
+def locator_section(L, nev=10000, cpu_events=16):
+    """Full grid search over 48 FP32 station tables built by the batched
+    solver and handed to the locator on the device."""
+    (nz, nx, ny, h), vel, stations = locator_workload()
+    ngrd = nz * nx * ny
+    models = [((nz, nx, ny), vel, s) for s in stations]
+    ms_tab, _, _ = run_batch(L, models, h, 2, 6)
+    ierr = c_int(1)
+    dp = ctypes.POINTER(c_double)
+    ip = ctypes.POINTER(c_int)
+    nsta = len(stations)
+    L.locate_initialize(nev, nsta, ngrd, byref(ierr))
+    assert ierr.value == 0
+    L.locate_setReferenceCompatibility(0, byref(ierr))
+    tables = np.zeros((nsta, ngrd))
+    for k in range(nsta):
+        L.locate_setTravelTimeFieldFromBatch3d(k + 1, k + 1, byref(ierr))
+        assert ierr.value == 0
+        tt = np.zeros(ngrd)
+        L.fteik_batch3d_getTravelTimeField64f(k + 1, ngrd, 0,
+                                              tt.ctypes.data_as(dp),
+                                              byref(ierr))
+        tables[k] = tt
+    L.fteik_batch3d_free()
+    hypo = np.random.default_rng(10).integers(0, ngrd, nev)
+    t0 = np.random.default_rng(11).uniform(0.0, 100.0, nev)
+    noise = np.random.default_rng(12).normal(0.0, 0.05, (nev, nsta))
+    picks = tables[:, hypo].T + t0[:, None] + noise
+    o2t = np.arange(1, nsta + 1, dtype=np.int32)
+    w = np.ones(nsta)
+    for e in range(nev):
+        p = np.ascontiguousarray(picks[e])
+        L.locate_setObservation64f(e + 1, nsta, False, 0.0,
+                                   o2t.ctypes.data_as(ip),
+                                   p.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                   byref(ierr))
+    ev = np.arange(1, nev + 1, dtype=np.int32)
+    opt = np.zeros(nev, dtype=np.int32)
+    t0o = np.zeros(nev)
+    obj = np.zeros(nev)
+    args = (nev, ev.ctypes.data_as(ip), opt.ctypes.data_as(ip),
+            t0o.ctypes.data_as(dp), obj.ctypes.data_as(dp), byref(ierr))
+    L.locate_locateEvents(*args)               # warm-up
+    hip_sync()
+    tstart = time.perf_counter()
+    L.locate_locateEvents(*args)
+    hip_sync()
+    tgpu = time.perf_counter() - tstart
+    assert ierr.value == 0
+    flops = 9.0 * nsta * ngrd * nev
+    out = {"workload": "48 stations, 161x161x81 h=250 m, v=4000+0.4z, "
+                       "10,000 events, sigma=0.05 s, full grid search",
+           "events": nev, "seconds": tgpu, "value": nev / tgpu,
+           "unit": "events/s", "tables_ms": ms_tab,
+           "achieved_TFLOPs": flops / tgpu / 1e12, "fp32_peak_TFLOPs": 157.3,
+           "argmin_equals_true_cell": float(np.mean(
+               np.abs(opt - 1 - hypo) == 0))}
+    # CPU baseline and parity on a sample of events (oracle, fresh state)
+    from oracle import oracle as orc
+    cores = min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    loc = orc.Locator(nev, nsta, ngrd)
+    for k in range(nsta):
+        loc.set_table(k + 1, tables[k])
+    same = 0
+    tcpu = 0.0
+    for e in range(cpu_events):
+        loc.set_observation(e + 1, o2t, picks[e], w)
+        tstart = time.perf_counter()
+        r = loc.locate(e + 1, accumulate_t0=False, pad_argmin=False)
+        tcpu += time.perf_counter() - tstart
+        same += r == (int(opt[e]), float(t0o[e]), float(obj[e]))
+    out["cpu_baseline"] = {"value": cpu_events / tcpu, "unit": "events/s",
+                           "cores": cores, "kind": "port",
+                           "sample": "%d events" % cpu_events}
+    out["gpu_equals_oracle_on_sample"] = same == cpu_events
+    L.locate_setReferenceCompatibility(1, byref(ierr))
+    L.locate_finalize()
+    return out
+
+
 def hip_sync():
     hip = ctypes.CDLL("libamdhip64.so.7")
     return hip.hipDeviceSynchronize()
@@ -74,6 +234,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched-solve and locator sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
@@ -234,6 +396,16 @@ def main():
                      "updates_per_launch": units},
         "cpu_baseline": cpu,
     }
+    if not args.no_extra:
+        nt = c_int()
+        t1, t2 = c_int(), c_int()
+        L.fteik_solver3d_getTile(byref(t1), byref(t2), byref(nt), byref(ierr))
+        out["batch"] = batch_section(L, nt.value)
+        print("bench: batch %.1f Gupd/s" % out["batch"]["value"],
+              file=sys.stderr, flush=True)
+        out["locator"] = locator_section(L)
+        print("bench: locator %.0f events/s" % out["locator"]["value"],
+              file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -128,6 +128,17 @@ def load_library(path=None):
         "fteik_amd_evalLocalSolver3d": [i, _dp, _ip, _dp, _dp, _ip],
         "fteik_amd_prepareSource3d": [i, _ip],
         "fteik_amd_timeSweeps3d": [i, i, _dp, _ip],
+        "fteik_batch3d_initialize": [i, _ip],
+        "fteik_batch3d_free": [],
+        "fteik_batch3d_setGrid": [i, i, i, i, d, d, d, d, d, d, _dp, d, d, d,
+                                  _ip],
+        "fteik_batch3d_setPrecision": [i, _ip],
+        "fteik_batch3d_setTile": [i, _ip],
+        "fteik_batch3d_solve": [i, d, _ip],
+        "fteik_batch3d_getLastSolveInfo": [_dp, _ip, _dp, _ip],
+        "fteik_batch3d_getTravelTimeField64f": [i, i, i, _dp, _ip],
+        "fteik_batch3d_getNumberOfGrids": [_ip, _ip],
+        "locate_setTravelTimeFieldFromBatch3d": [i, i, _ip],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

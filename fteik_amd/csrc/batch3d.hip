@@ -1,0 +1,135 @@
+// batch3d.hip -- many single-source 3D solves, each on its own grid, swept
+// together: one launch per sweep direction covers every tile of every grid.
+//
+// The reference solves one source at a time (solver3D.F90:1156-1243); here a
+// batch of (grid, source) pairs of uneven sizes is advanced sweep by sweep
+// with the same pipelined column-tile body as the single-grid solver
+// (sweep_pencil.h).  Workgroup ids are ordered by tile diagonal first
+// (a + b of the tile's flipped position), then grid, then a: every tile's
+// upwind producers in its own grid have lower ids, and the first tiles of
+// all grids start together.  Grids do not interact.
+//
+// The launch is not cooperative (the batch is larger than the GPU), so
+// forward progress rests on in-order dispatch: a waiting workgroup only
+// waits on lower ids, which were dispatched before it on their own XCD and
+// are running or done.  Every wait keeps the spin limit of the body.
+#include <hip/hip_runtime.h>
+#include "sweep_pencil.h"
+
+namespace fteik {
+
+struct BatchTile { int grid, tile, flag, pad; };
+
+template <typename R, bool INIT, int BX, int BY, int HB>
+__global__ __launch_bounds__(BX*BY)
+void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
+                        const BatchTile* __restrict__ map, int sweep,
+                        unsigned long long serial,
+                        unsigned long long* __restrict__ flags,
+                        int* __restrict__ err, int fp32, int track)
+{
+    const BatchTile m = map[blockIdx.x];
+    const BatchGrid& g = grids[m.grid];
+    SweepArgs<R> a;
+    a.tt = (R*) (fp32 ? g.tt32 : g.tt);
+    a.slow = (const R*) (fp32 ? g.slow32 : g.slow);
+    a.cst = (const Consts<R>*) (fp32 ? g.cst32 : g.cst);
+    a.nz = g.nz; a.nx = g.nx; a.ny = g.ny;
+    sweep_signs(sweep, a.sz, a.sx, a.sy);
+    int b[6];
+    flipped_box(sweep, INIT, g.nz, g.nx, g.ny, g.zsi, g.xsi, g.ysi, b);
+    a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
+    a.fy0 = b[4]; a.fy1 = b[5];
+    a.nbx = (g.nx + BX - 1)/BX;
+    a.nby = (g.ny + BY - 1)/BY;
+    // this tile's flag is flags[m.flag]; the body addresses flags by the
+    // grid-local tile number, so hand it the grid's base
+    a.flags = flags + (m.flag - m.tile);
+    a.err = err;
+    a.serial = serial;
+    a.maxdelta = track ? g.maxdelta : nullptr;
+    if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(flags + m.flag, (serial << 32) | 0xffffffffull,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    pencil_body<R, INIT, BX, BY, HB>(a, m.tile);
+}
+
+template <typename R, bool INIT>
+static const void* batch_fn_t(int tile)
+{
+#define BF(n) case n: return (const void*) batch_sweep_kernel<R, INIT, \
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
+    switch (tile) {
+        BF(0) BF(1) BF(2) BF(3) BF(4) BF(5) BF(6)
+        default: return nullptr;
+    }
+#undef BF
+}
+static_assert(kNumTiles == 7, "batch_fn_t lists every tile");
+
+} // namespace fteik
+
+using namespace fteik;
+
+// Workgroup order of a batch for tile shape `tile`: tiles sorted by
+// diagonal, then grid, then x; flag slots are grid-major.  map receives
+// 4 ints per workgroup; returns the workgroup count (or -1 if map_cap is too
+// small).  Host-side bookkeeping only.
+extern "C" int fteik_k_batch_map(int tile, int ngrids, const int* dims,
+                                 int* map, int map_cap, int* nflags)
+{
+    if (tile < 0 || tile >= kNumTiles) return -1;
+    const TileShape ts = kTiles[tile];
+    int total = 0, maxdiag = 0;
+    for (int g = 0; g < ngrids; g++) {
+        const int nbx = (dims[3*g + 1] + ts.bx - 1)/ts.bx;
+        const int nby = (dims[3*g + 2] + ts.by - 1)/ts.by;
+        total += nbx*nby;
+        maxdiag = (nbx + nby - 2 > maxdiag) ? nbx + nby - 2 : maxdiag;
+    }
+    *nflags = total;
+    if (total > map_cap) return -1;
+    int w = 0;
+    for (int d = 0; d <= maxdiag; d++) {
+        int base = 0;
+        for (int g = 0; g < ngrids; g++) {
+            const int nbx = (dims[3*g + 1] + ts.bx - 1)/ts.bx;
+            const int nby = (dims[3*g + 2] + ts.by - 1)/ts.by;
+            for (int bx = 0; bx < nbx; bx++) {
+                const int by = d - bx;
+                if (by < 0 || by >= nby) continue;
+                map[4*w] = g;
+                map[4*w + 1] = bx*nby + by;
+                map[4*w + 2] = base + bx*nby + by;
+                map[4*w + 3] = 0;
+                w++;
+            }
+            base += nbx*nby;
+        }
+    }
+    return w;
+}
+
+extern "C" int fteik_k_batch_sweep(const BatchGrid* grids, const int* map,
+                                   int nwg, int tile, int sweep, int linit,
+                                   unsigned long long serial,
+                                   unsigned long long* flags, int* err,
+                                   int fp32, int track, hipStream_t stream)
+{
+    if (tile < 0 || tile >= kNumTiles || nwg < 1) return 2;
+    const TileShape ts = kTiles[tile];
+    const void* fn = fp32 ? (linit ? batch_fn_t<float, true>(tile)
+                                   : batch_fn_t<float, false>(tile))
+                          : (linit ? batch_fn_t<double, true>(tile)
+                                   : batch_fn_t<double, false>(tile));
+    const BatchTile* m = (const BatchTile*) map;
+    void* args[] = {(void*) &grids, (void*) &m, (void*) &sweep,
+                    (void*) &serial, (void*) &flags, (void*) &err,
+                    (void*) &fp32, (void*) &track};
+    hipError_t e = hipLaunchKernel(fn, dim3(nwg), dim3(ts.bx*ts.by), args, 0,
+                                   stream);
+    return e == hipSuccess ? 0 : 1;
+}

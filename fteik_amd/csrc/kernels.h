@@ -30,9 +30,24 @@ struct SweepLaunch {
     int tile;                    // tile shape index of the pipelined sweep
 };
 
+// One grid of a batched solve (batch3d.hip); device pointers.
+struct BatchGrid {
+    void* tt; const void* slow; const void* cst;        // FP64
+    void* tt32; const void* slow32; const void* cst32;  // FP32 (mixed)
+    double* maxdelta;                                   // per-grid monitor
+    int nz, nx, ny, zsi, xsi, ysi;
+};
+
 } // namespace fteik
 
 extern "C" {
+// batch3d.hip
+int fteik_k_batch_map(int tile, int ngrids, const int* dims, int* map,
+                      int map_cap, int* nflags);
+int fteik_k_batch_sweep(const fteik::BatchGrid* grids, const int* map,
+                        int nwg, int tile, int sweep, int linit,
+                        unsigned long long serial, unsigned long long* flags,
+                        int* err, int fp32, int track, hipStream_t stream);
 // sweep3d.hip
 int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
                     hipStream_t stream);

@@ -1,69 +1,14 @@
-// sweep3d.hip -- 3D level-set fast sweeping on gfx950.
-//
-// One sweep of the reference (autoCode.f90:1001-1316 and siblings: every
-// masked node updated once, level by level, levels = flipped iz+ix+iy) runs
-// here as one kernel launch organised as a pipelined column wavefront:
-//
-//   * the (x, y) plane of the sweep's flipped frame is cut into blocks of
-//     16 x 16 columns; one 256-thread workgroup owns a block, one thread owns
-//     a column and walks down it, one node per step;
-//   * at step t every column processes the node on global level t, so all
-//     nodes a node depends on were produced at steps t-1..t-3; values of the
-//     last four steps live in an LDS ring, one barrier per step;
-//   * the upwind block neighbours (x-1, y-1, corner) publish a progress
-//     counter every H steps; a block pulls its halo columns for the next H
-//     steps with coherent loads once its producers are far enough ahead.
-//     Producers are always geometrically 16 steps ahead, so in steady state
-//     nobody waits and the whole sweep takes ~nx+ny+nz steps;
-//   * blocks are numbered upwind-first, so every dependency points to a
-//     lower workgroup id (in-order dispatch cannot deadlock); launches use
-//     cooperative launch whenever the grid fits, and every wait has a spin
-//     limit that turns a hang into a reported error.
-//
-// Kernel boundaries between sweeps give the coherence the next sweep needs.
+// sweep3d.hip -- 3D level-set fast sweeping on gfx950: the single-grid
+// solver's kernels and launchers.  One sweep direction is one launch of the
+// pipelined column-tile kernel (sweep_pencil.h); kernel boundaries between
+// sweeps give the coherence the next sweep needs.  Launches are cooperative
+// (every tile resident) so waits between tiles always end; schedule 1 is a
+// one-launch-per-level cross-check and fallback.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "stencils.h"
-#include "kernels.h"
+#include "sweep_pencil.h"
 
 namespace fteik {
-
-// Tile shapes of the pipelined sweep: BX x BY columns per workgroup, halo
-// exchanged with the upwind workgroups every HB steps.  The autotuner picks
-// one per grid (fteik_solver3d_autotune); kTiles[0] is the default.
-struct TileShape { int bx, by, hb; };
-constexpr TileShape kTiles[] = {
-    {16, 16, 8}, {16, 16, 4}, {16, 16, 2}, {8, 8, 4}, {8, 8, 2}, {32, 8, 4},
-    {16, 8, 4},
-};
-constexpr int kNumTiles = (int) (sizeof(kTiles)/sizeof(kTiles[0]));
-constexpr unsigned long long kSpinLimit = 1ull << 22;
-
-// ---------------------------------------------------------------- utils
-__device__ __forceinline__ void store_coherent(double* p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_coherent(float* p, float v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_coherent(const double* p)
-{
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float load_coherent(const float* p)
-{
-    return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Flipped 0-based coordinate -> actual 1-based index.
-__device__ __forceinline__ int unflip(int f, int n, int sgn)
-{
-    return (sgn > 0) ? f + 1 : n - f;
-}
 
 template <typename R>
 __global__ void fill_kernel(R* __restrict__ a, long n, R v)
@@ -140,301 +85,11 @@ __global__ void consts_to_float_kernel(const Consts<double>* __restrict__ cd,
     *cf = b;
 }
 
-// ------------------------------------------------------- pencil sweep
-template <typename R>
-struct SweepArgs {
-    R* tt;
-    const R* slow;
-    const Consts<R>* cst;
-    unsigned long long* flags;   // one per block, (serial << 32) | step+1
-    int* err;
-    unsigned long long serial;
-    int nz, nx, ny;
-    int sz, sx, sy;              // time signs of the sweep
-    int fz0, fz1, fx0, fx1, fy0, fy1;   // update box, flipped 0-based
-    int nbx, nby;                // blocks along flipped x, y
-    double* maxdelta;            // optional: max |t_new - t_old| (as bits)
-};
-
-template <typename R>
-__device__ __forceinline__ void slowness7(const R* __restrict__ slow, int nz,
-                                          int nx, int ny, int i, int j, int k,
-                                          int sz, int sx, int sy, Slw<R>& s)
-{
-    // autoCode.f90:407-445: all 19 cells come from the clamped 2x2x2 cell
-    // neighbourhood; i1 = i - sgnv is im for + sweeps and ip for - sweeps.
-    const int im = max(i - 1, 1), ip = min(i, nz - 1);
-    const int jm = max(j - 1, 1), jp = min(j, nx - 1);
-    const int km = max(k - 1, 1), kp = min(k, ny - 1);
-    const long sz1 = nz - 1, szx = (long) (nz - 1)*(nx - 1);
-#define SC(a, b, c) slow[(long) ((c) - 1)*szx + (long) ((b) - 1)*sz1 + (a) - 1]
-    const R c000 = SC(im, jm, km), c100 = SC(ip, jm, km);
-    const R c010 = SC(im, jp, km), c110 = SC(ip, jp, km);
-    const R c001 = SC(im, jm, kp), c101 = SC(ip, jm, kp);
-    const R c011 = SC(im, jp, kp), c111 = SC(ip, jp, kp);
-#undef SC
-    // select by the sweep's sgnv: index 0 -> (i-1 side), 1 -> (i side)
-    const bool zi = (sz < 0), xj = (sx < 0), yk = (sy < 0);
-    // cells in plane i1 over (j, k)
-    const R a00 = zi ? c100 : c000, a10 = zi ? c110 : c010;
-    const R a01 = zi ? c101 : c001, a11 = zi ? c111 : c011;
-    s.s1 = rmin(rmin(rmin(a00, a01), a10), a11);
-    // cells in plane j1 over (i, k)
-    const R b00 = xj ? c010 : c000, b10 = xj ? c110 : c100;
-    const R b01 = xj ? c011 : c001, b11 = xj ? c111 : c101;
-    s.s2 = rmin(rmin(rmin(b00, b10), b01), b11);
-    // cells in plane k1 over (i, j)
-    const R d00 = yk ? c001 : c000, d10 = yk ? c101 : c100;
-    const R d01 = yk ? c011 : c010, d11 = yk ? c111 : c110;
-    s.s3 = rmin(rmin(rmin(d00, d01), d10), d11);
-    // line (i1, j1) over k
-    const R e0 = zi ? (xj ? c110 : c100) : (xj ? c010 : c000);
-    const R e1 = zi ? (xj ? c111 : c101) : (xj ? c011 : c001);
-    s.s4 = rmin(e0, e1);
-    // line (i1, k1) over j
-    const R f0 = zi ? (yk ? c101 : c100) : (yk ? c001 : c000);
-    const R f1 = zi ? (yk ? c111 : c110) : (yk ? c011 : c010);
-    s.s5 = rmin(f0, f1);
-    // line (j1, k1) over i
-    const R g0 = xj ? (yk ? c011 : c010) : (yk ? c001 : c000);
-    const R g1 = xj ? (yk ? c111 : c110) : (yk ? c101 : c100);
-    s.s6 = rmin(g0, g1);
-    // cell (i1, j1, k1)
-    s.s7 = zi ? (xj ? (yk ? c111 : c110) : (yk ? c101 : c100))
-              : (xj ? (yk ? c011 : c010) : (yk ? c001 : c000));
-}
-
-// The 7 min-reduced slownesses from the node's 2x2x2 cell neighbourhood
-// given as u (upwind, the i1/j1/k1 side of autoCode.f90:407-409) and d
-// (other side) along each axis: c[z][x][y], index 0 = u, 1 = d.
-template <typename R>
-__device__ __forceinline__ void slowness7_ud(const R (&zu)[4], const R (&zd)[4],
-                                             Slw<R>& s)
-{
-    // zu/zd hold (x,y) = (u,u), (u,d), (d,u), (d,d)
-    s.s1 = rmin(rmin(rmin(zu[0], zu[1]), zu[2]), zu[3]);       // plane i1
-    s.s2 = rmin(rmin(rmin(zu[0], zd[0]), zu[1]), zd[1]);       // plane j1
-    s.s3 = rmin(rmin(rmin(zu[0], zu[2]), zd[0]), zd[2]);       // plane k1
-    s.s4 = rmin(zu[0], zu[1]);                                 // line i1,j1
-    s.s5 = rmin(zu[0], zu[2]);                                 // line i1,k1
-    s.s6 = rmin(zu[0], zd[0]);                                 // line j1,k1
-    s.s7 = zu[0];                                              // cell
-}
-
 template <typename R, bool INIT, int BX, int BY, int HB>
 __global__ __launch_bounds__(BX*BY)
 void sweep3d_pencil_kernel(SweepArgs<R> a)
 {
-    constexpr int NT = BX*BY;
-    constexpr int RING = (HB + 3 <= 8) ? 8 : 16;     // LDS ring depth
-    constexpr int HCOLS = BX + BY + 1;
-    static_assert(HB + 3 <= RING, "ring too shallow");
-    __shared__ R ring[RING][(BX + 1)*(BY + 1)];
-    __shared__ int s_abort;
-    const int wg = blockIdx.x;
-    const int fbx = wg / a.nby, fby = wg - (wg / a.nby)*a.nby;
-    const int fj0 = fbx*BX, fk0 = fby*BY;
-    const int lj = threadIdx.x % BX, lk = threadIdx.x / BX;
-    const int fj = fj0 + lj, fk = fk0 + lk;
-    const int nz = a.nz, nx = a.nx, ny = a.ny;
-    // extended box: update box plus the one frozen layer its nodes read
-    const int ez0 = a.fz0 - 1, ex0 = a.fx0 - 1, ey0 = a.fy0 - 1;
-    const int jlo = max(fj0, ex0), jhi = min(min(fj0 + BX - 1, a.fx1), nx - 1);
-    const int klo = max(fk0, ey0), khi = min(min(fk0 + BY - 1, a.fy1), ny - 1);
-    unsigned long long* myflag = a.flags + wg;
-    const unsigned long long hi = a.serial << 32;
-    if (jlo > jhi || klo > khi) {
-        if (threadIdx.x == 0)
-            __hip_atomic_store(myflag, hi | 0xffffffffull, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        return;
-    }
-    const int tlo = ez0 + jlo + klo;
-    const int thi = a.fz1 + jhi + khi;
-    const bool colE = (fj >= jlo && fj <= jhi && fk >= klo && fk <= khi);
-    const bool colB = colE && fj >= a.fx0 && fk >= a.fy0;
-    // Loads of threads outside the extended box use a clamped in-grid
-    // column; their values are never used.
-    const int cj = min(max(fj, jlo), jhi), ck = min(max(fk, klo), khi);
-    const int j = unflip(cj, nx, a.sx), k = unflip(ck, ny, a.sy);
-    R* const col = a.tt + (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
-    // the column's four cell columns (autoCode.f90:407-445 clamps)
-    const int jm = max(j - 1, 1), jp = min(j, nx - 1);
-    const int km = max(k - 1, 1), kp = min(k, ny - 1);
-    const int xu = (a.sx > 0) ? jm : jp, xd = (a.sx > 0) ? jp : jm;
-    const int yu = (a.sy > 0) ? km : kp, yd = (a.sy > 0) ? kp : km;
-    const long szx = (long) (nz - 1)*(nx - 1);
-    const R* const c_uu = a.slow + (long) (yu - 1)*szx + (long) (xu - 1)*(nz - 1);
-    const R* const c_ud = a.slow + (long) (yd - 1)*szx + (long) (xu - 1)*(nz - 1);
-    const R* const c_du = a.slow + (long) (yu - 1)*szx + (long) (xd - 1)*(nz - 1);
-    const R* const c_dd = a.slow + (long) (yd - 1)*szx + (long) (xd - 1)*(nz - 1);
-    const int fdep = cj + ck;           // node of step t is fi = t - fdep
-    const Consts<R> c = *a.cst;
-    const int me = (lk + 1)*(BX + 1) + (lj + 1);
-    const int cW = me - 1, cS = me - (BX + 1), cSW = me - (BX + 2);
-    R dmax = R(0);
-
-    // 0-based offset of flipped node fi in its column, clamped to the box
-    auto node_off = [&](int fi) -> int {
-        fi = min(max(fi, ez0), a.fz1);
-        return (a.sz > 0) ? fi : nz - 1 - fi;
-    };
-    // 0-based z offset of the cell layer on the upwind side of flipped
-    // node f (its i1 plane); the downwind layer of f is the upwind one of
-    // f + 1.  Clamped as max(i-1,1) / min(i,nz-1) clamp the cell index.
-    auto layer_off = [&](int f) -> int {
-        const int zc = (a.sz > 0) ? f : nz - f;          // 1-based cell
-        return min(max(zc, 1), nz - 1) - 1;
-    };
-
-    // Register double buffer: own old times and cell layers of the next
-    // HB steps are loaded one batch ahead, so no step waits on memory.
-    R tn[HB], cn[HB + 1][4];
-    auto prefetch = [&](int tb) {
-        const int f0 = tb - fdep;
-        #pragma unroll
-        for (int q = 0; q < HB; q++) tn[q] = col[node_off(f0 + q)];
-        #pragma unroll
-        for (int q = 0; q <= HB; q++) {
-            const int z = layer_off(f0 + q);
-            cn[q][0] = c_uu[z]; cn[q][1] = c_ud[z];
-            cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
-        }
-    };
-
-    if (threadIdx.x == 0) {
-        s_abort = 0;
-        __hip_atomic_store(myflag, hi | (unsigned long long) tlo,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // The initialization sweeps (heavier stencil, once per solve) load each
-    // batch at its start instead, which keeps their register count low
-    // enough for two workgroups per CU.
-    if (!INIT) prefetch(tlo);
-    __syncthreads();
-
-    for (int tb = tlo; tb <= thi; tb += HB) {
-        if (INIT) prefetch(tb);
-        R tc[HB], cc[HB + 1][4];
-        #pragma unroll
-        for (int q = 0; q < HB; q++) tc[q] = tn[q];
-        #pragma unroll
-        for (int q = 0; q <= HB; q++) {
-            cc[q][0] = cn[q][0]; cc[q][1] = cn[q][1];
-            cc[q][2] = cn[q][2]; cc[q][3] = cn[q][3];
-        }
-        if (!INIT && tb + HB <= thi) prefetch(tb + HB);
-
-        // halo batch: steps tb-3 .. tb+HB-2 of the upwind columns (the three
-        // oldest are needed by the first batch only)
-        {
-            const int need = tb + HB - 2;
-            if (threadIdx.x < 3) {
-                int pbx = fbx - (threadIdx.x != 1 ? 1 : 0);
-                int pby = fby - (threadIdx.x != 0 ? 1 : 0);
-                if (pbx >= 0 && pby >= 0) {
-                    const unsigned long long* pf = a.flags + pbx*a.nby + pby;
-                    const unsigned long long want =
-                        hi | (unsigned long long) (need + 1);
-                    unsigned long long spins = 0;
-                    while (__hip_atomic_load(pf, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) < want) {
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++spins > kSpinLimit) {
-                            atomicExch(a.err, 1);
-                            s_abort = 1;
-                            break;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            if (s_abort) {
-                // release the blocks that wait on this one; the error flag
-                // marks the whole sweep as failed
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(myflag, hi | 0xffffffffull,
-                                       __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                return;
-            }
-            for (int h = threadIdx.x; h < HCOLS*(HB + 2); h += NT) {
-                int hc = h / (HB + 2), d = h - (h / (HB + 2))*(HB + 2);
-                int st = tb - 3 + d;
-                int hj, hk, slot;
-                if (hc < BY)            { hj = fj0 - 1; hk = fk0 + hc;
-                                          slot = (hc + 1)*(BX + 1); }
-                else if (hc < BY + BX)  { hj = fj0 + (hc - BY); hk = fk0 - 1;
-                                          slot = hc - BY + 1; }
-                else                    { hj = fj0 - 1; hk = fk0 - 1; slot = 0; }
-                int fi = st - hj - hk;
-                if (hj >= 0 && hk >= 0 && hj < nx && hk < ny && fi >= 0 &&
-                    fi < nz) {
-                    int i = unflip(fi, nz, a.sz);
-                    int jj = unflip(hj, nx, a.sx), kk = unflip(hk, ny, a.sy);
-                    long g = (long) (kk - 1)*nz*nx + (long) (jj - 1)*nz + i - 1;
-                    ring[st & (RING - 1)][slot] = load_coherent(a.tt + g);
-                }
-            }
-            __syncthreads();
-        }
-
-        #pragma unroll
-        for (int q = 0; q < HB; q++) {
-            const int t = tb + q;
-            if (t > thi) break;                       // block-uniform
-            const int fi = t - fj - fk;
-            if (colE && fi >= ez0 && fi <= a.fz1) {
-                R v = tc[q];
-                if (colB && fi >= a.fz0) {
-                    const int s1 = (t - 1) & (RING - 1),
-                              s2 = (t - 2) & (RING - 1),
-                              s3 = (t - 3) & (RING - 1);
-                    Nbr<R> nb;
-                    nb.tv   = ring[s1][me];
-                    nb.te   = ring[s1][cW];
-                    nb.tn   = ring[s1][cS];
-                    nb.tev  = ring[s2][cW];
-                    nb.ten  = ring[s2][cSW];
-                    nb.tnv  = ring[s2][cS];
-                    nb.tnve = ring[s3][cSW];
-                    Slw<R> sl;
-                    slowness7_ud<R>(cc[q], cc[q + 1], sl);
-                    R vn;
-                    if (INIT) {
-                        const int i = unflip(fi, nz, a.sz);
-                        vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
-                    } else {
-                        vn = noinit3d(c, nb, sl, v);
-                    }
-                    dmax = rmax(dmax, v - vn);
-                    v = vn;
-                    store_coherent(col + node_off(fi), v);
-                }
-                ring[t & (RING - 1)][me] = v;
-            }
-            if (q == HB - 1 || t == thi) {
-                __builtin_amdgcn_s_waitcnt(0);
-                __syncthreads();
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(myflag,
-                                       hi | (unsigned long long) (t + 1),
-                                       __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __syncthreads();
-            }
-        }
-    }
-    if (threadIdx.x == 0)
-        __hip_atomic_store(myflag, hi | 0xffffffffull, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    if (a.maxdelta && dmax > R(0)) {
-        // positive doubles order like their bit patterns
-        double dd = (double) dmax;
-        atomicMax((unsigned long long*) a.maxdelta,
-                  (unsigned long long) __double_as_longlong(dd));
-    }
+    pencil_body<R, INIT, BX, BY, HB>(a, blockIdx.x);
 }
 
 // Simple reference schedule: one launch per level (used for cross-checks
@@ -478,34 +133,6 @@ void sweep3d_level_kernel(SweepArgs<R> a, int L)
 }
 
 // ------------------------------------------------------------ launchers
-static void sweep_signs(int sweep, int& sz, int& sx, int& sy)
-{
-    // prefetch.py:4-8
-    static const int tz[8] = { 1,  1,  1,  1, -1, -1, -1, -1};
-    static const int tx[8] = { 1, -1,  1, -1,  1, -1,  1, -1};
-    static const int ty[8] = { 1,  1, -1, -1,  1,  1, -1, -1};
-    sz = tz[sweep - 1]; sx = tx[sweep - 1]; sy = ty[sweep - 1];
-}
-
-// solver3D.F90:493-565 box, converted to the flipped 0-based frame.
-static void flipped_box(int sweep, bool linit, int nz, int nx, int ny,
-                        int zsi, int xsi, int ysi, int* b)
-{
-    int sz, sx, sy;
-    sweep_signs(sweep, sz, sx, sy);
-    auto axis = [&](int sgn, int n, int si, int* lo, int* hi) {
-        // actual 1-based limits, then flip
-        int a0, a1;
-        if (sgn > 0) { a0 = linit ? max(2, si) : 2; a1 = n; }
-        else         { a0 = 1; a1 = linit ? si + 1 : n - 1; }
-        if (sgn > 0) { *lo = a0 - 1; *hi = a1 - 1; }
-        else         { *lo = n - a1; *hi = n - a0; }
-    };
-    axis(sz, nz, zsi, &b[0], &b[1]);
-    axis(sx, nx, xsi, &b[2], &b[3]);
-    axis(sy, ny, ysi, &b[4], &b[5]);
-}
-
 template <typename R, bool INIT>
 static const void* pencil_fn_t(int tile)
 {

@@ -198,6 +198,33 @@ void locate_setReferenceCompatibility(int mode, int *ierr);
 void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
                          double *t0Opt, double *objOpt, int *ierr);
 
+/* ========================= Batched 3D solver ========================= */
+/* EXTENSION (no counterpart in the reference, which solves one source at a
+ * time, solver3D.F90:1156): a batch of ngrids independent single-source
+ * solves, each on its own grid (sizes may differ), advanced together: one
+ * launch per sweep direction covers every grid.  Each grid follows
+ * fteik_solver3d_solveSourceLSM exactly (FP64 fields are bitwise those of
+ * the single-grid solver), or the mixed mode of fteik_solver3d_setPrecision.
+ * vel holds (nz-1)(nx-1)(ny-1) cell velocities in natural order. */
+void fteik_batch3d_initialize(int ngrids, int *ierr);
+void fteik_batch3d_free(void);
+void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
+                           double x0, double y0, double dz, double dx,
+                           double dy, const double *vel, double zs, double xs,
+                           double ys, int *ierr);
+void fteik_batch3d_setPrecision(int mode, int *ierr);
+/* tile shape for the whole batch (see fteik_solver3d_setTile) */
+void fteik_batch3d_setTile(int tile, int *ierr);
+void fteik_batch3d_solve(int nsweep, double eps, int *ierr);
+void fteik_batch3d_getLastSolveInfo(double *ms, int *corrIterations,
+                                    double *updates, int *ierr);
+void fteik_batch3d_getTravelTimeField64f(int igrid, int ngrd, int order,
+                                         double *tt, int *ierr);
+void fteik_batch3d_getNumberOfGrids(int *ngrids, int *ierr);
+/* EXTENSION: locator table itf (1-based) from batch grid igrid without a
+ * host round trip; same rounding as locate_setTravelTimeField64f. */
+void locate_setTravelTimeFieldFromBatch3d(int igrid, int itf, int *ierr);
+
 /* =========================== Model helpers =========================== */
 /* model.F90:771 */
 int fteik_model_grid2indexF(int i, int j, int k, int nz, int nzx);
