@@ -139,6 +139,18 @@ def load_library(path=None):
         "fteik_batch3d_getTravelTimeField64f": [i, i, i, _dp, _ip],
         "fteik_batch3d_getNumberOfGrids": [_ip, _ip],
         "locate_setTravelTimeFieldFromBatch3d": [i, i, _ip],
+        "fteik_surrogate_initialize": [i, i, i, i, d, d, d, _dp, _dp, _dp, d,
+                                       d, i, _ip],
+        "fteik_surrogate_free": [],
+        "fteik_surrogate_getNumberOfParameters": [_ip, _ip],
+        "fteik_surrogate_getParameters": [i, _fp, _ip],
+        "fteik_surrogate_setParameters": [i, _fp, _ip],
+        "fteik_surrogate_forward": [i, _fp, _fp, _ip],
+        "fteik_surrogate_gradients": [i, _fp, _fp, _fp, _dp, _ip],
+        "fteik_surrogate_train": [i, i, d, i, _dp, _ip],
+        "locate_locateEventsSurrogate": [i, _ip, i, i, _ip, _dp, _dp, _ip,
+                                         _ip],
+        "fteik_amd_gemmBf16": [i, i, i, _fp, _fp, _fp, _ip],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -133,3 +133,55 @@ int fteik_k_locate_t0_at(float* t0, const float* tables, int ntf, long ldgrd,
                          int lntf, const int* cells, int nev, int ngrd,
                          hipStream_t stream);
 }
+
+extern "C" {
+// surrogate.hip
+int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
+                      long sbk, long sbn, float* C, long ldc, int M, int N,
+                      int K, const float* bias, int accumulate,
+                      hipStream_t stream);
+int fteik_k_f2bf(const float* a, void* b, long n, hipStream_t stream);
+int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
+                       float* z, void* h, long rows, int H, hipStream_t stream);
+int fteik_k_sur_silu(const float* z, void* h, long n, hipStream_t stream);
+int fteik_k_sur_silu_bwd(const float* dh, const float* z, void* dz, long n,
+                         hipStream_t stream);
+int fteik_k_sur_mse(const float* y, const float* tgt, void* dy, long n,
+                    float inv, double* loss, hipStream_t stream);
+int fteik_k_sur_colsum(const void* d, long rows, int cols, float* out,
+                       hipStream_t stream);
+int fteik_k_sur_dw1(const void* dz, const float* x, long rows, int H,
+                    float* dW1, hipStream_t stream);
+int fteik_k_sur_adam(float* p, const float* g, float* m, float* v, long n,
+                     float lr, float b1, float b2, float eps, float c1,
+                     float c2, void* pb, hipStream_t stream);
+int fteik_k_sur_samples(const float* tab, long ldgrd, const void* geo,
+                        long ngrd, unsigned long long seed, long pool,
+                        long first, long rows, float* x, float* tgt, int ldt,
+                        hipStream_t stream);
+int fteik_k_sur_geo_size(void);
+int fteik_k_inv_start(float* pos, long R, int starts, hipStream_t stream);
+int fteik_k_inv_layer1(const float* pos, long R, const float* W1,
+                       const float* b1, float* zp, void* h, int H,
+                       hipStream_t stream);
+int fteik_k_inv_act(const float* zall, long R, int H, const float* b,
+                    float* zp, void* h, hipStream_t stream);
+int fteik_k_inv_gn(const float* y, long R, int ldy, const float* b5,
+                   const void* geo, const float* obs, const float* wt,
+                   int lntf, int starts, float* pos, float* misfit, int apply,
+                   hipStream_t stream);
+int fteik_k_inv_score(const float* pos, const float* misfit, int starts,
+                      int nev, const void* geo, const float* tab, long ldgrd,
+                      const float* obs, const float* wt,
+                      const unsigned char* skip, const float* xnorm, int lntf,
+                      unsigned long long* best, int* fallback,
+                      hipStream_t stream);
+}
+
+// Host mirror of the surrogate's geometry block (surrogate.hip GeoArgs).
+struct FteikSurrogateGeo {
+    int nz, nx, ny, ntf;
+    float dz, dx, dy;
+    float sx[64], sy[64], sz[64];
+    float v0, scale;
+};

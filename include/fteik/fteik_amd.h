@@ -225,6 +225,40 @@ void fteik_batch3d_getNumberOfGrids(int *ngrids, int *ierr);
  * host round trip; same rounding as locate_setTravelTimeField64f. */
 void locate_setTravelTimeFieldFromBatch3d(int igrid, int itf, int *ierr);
 
+/* ===================== Learned locator surrogate ===================== */
+/* EXTENSION (no counterpart in the reference): an MLP 3 -> 256 -> 256 ->
+ * 256 -> 256 -> ntf (SiLU) from a normalised hypocentre to the stations'
+ * travel times, trained on the GPU (BF16 MFMA, FP32 accumulation, Adam) on
+ * cells sampled from the locator's tables.  It predicts
+ * (T - |x - station|/v0)/scale.  Stations are given in metres relative to
+ * the grid origin.  Positions are normalised to [-1, 1]^3 as (x, y, z). */
+void fteik_surrogate_initialize(int ntf, int nz, int nx, int ny, double dz,
+                                double dx, double dy, const double *zsta,
+                                const double *xsta, const double *ysta,
+                                double v0, double scale, int seed, int *ierr);
+void fteik_surrogate_free(void);
+void fteik_surrogate_getNumberOfParameters(int *np, int *ierr);
+void fteik_surrogate_getParameters(int np, float *p, int *ierr);
+void fteik_surrogate_setParameters(int np, const float *p, int *ierr);
+/* network outputs (before baseline and scale) at n positions */
+void fteik_surrogate_forward(int n, const float *pos, float *out, int *ierr);
+/* one forward/backward pass: gradients of the mean squared error */
+void fteik_surrogate_gradients(int n, const float *pos, const float *tgt,
+                               float *grad, double *loss, int *ierr);
+/* training on the locator's tables; lossOut = mean loss of the last steps */
+void fteik_surrogate_train(int steps, int batch, double lr, int pool,
+                           double *lossOut, int *ierr);
+/* surrogate-pruned location (fresh origin-time state): Gauss-Newton on the
+ * surrogate's misfit from `starts` (<= 9) points, exact scoring of the
+ * 4 x 4 x 4 cells around the best, full grid search for events whose best
+ * cell lies on the cube's faces (counted in nFallback). */
+void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
+                                  int iterations, int *optIndx, double *t0Opt,
+                                  double *objOpt, int *nFallback, int *ierr);
+/* diagnostic: C = A B (row-major FP32 in/out) through the BF16 MFMA GEMM */
+void fteik_amd_gemmBf16(int M, int N, int K, const float *A, const float *B,
+                        float *C, int *ierr);
+
 /* =========================== Model helpers =========================== */
 /* model.F90:771 */
 int fteik_model_grid2indexF(int i, int j, int k, int nz, int nzx);
