@@ -112,7 +112,12 @@ def batch_section(L, ntiles):
     h = 50.0
     models = batch_models()
     times = {}
+    ierr = c_int(1)
+    L.fteik_batch3d_initialize(1, byref(ierr))
     for tile in range(ntiles):
+        L.fteik_batch3d_setTile(tile, byref(ierr))
+        if ierr.value != 0:          # shapes the batched solver does not use
+            continue
         ms, upd, _ = run_batch(L, models, h, 2, tile)
         times[tile] = ms
     best = min(times, key=times.get)
@@ -141,7 +146,7 @@ def locator_workload():
     return (nz, nx, ny, h), vel, stations
 
 
-def locator_section(L, nev=10000, cpu_events=16):
+def locator_section(L, args, nev=10000, cpu_events=16):
     """Full grid search over 48 FP32 station tables built by the batched
     solver and handed to the locator on the device."""
     (nz, nx, ny, h), vel, stations = locator_workload()
@@ -181,12 +186,12 @@ def locator_section(L, nev=10000, cpu_events=16):
     opt = np.zeros(nev, dtype=np.int32)
     t0o = np.zeros(nev)
     obj = np.zeros(nev)
-    args = (nev, ev.ctypes.data_as(ip), opt.ctypes.data_as(ip),
+    call = (nev, ev.ctypes.data_as(ip), opt.ctypes.data_as(ip),
             t0o.ctypes.data_as(dp), obj.ctypes.data_as(dp), byref(ierr))
-    L.locate_locateEvents(*args)               # warm-up
+    L.locate_locateEvents(*call)               # warm-up
     hip_sync()
     tstart = time.perf_counter()
-    L.locate_locateEvents(*args)
+    L.locate_locateEvents(*call)
     hip_sync()
     tgpu = time.perf_counter() - tstart
     assert ierr.value == 0
@@ -217,9 +222,63 @@ def locator_section(L, nev=10000, cpu_events=16):
                            "cores": cores, "kind": "port",
                            "sample": "%d events" % cpu_events}
     out["gpu_equals_oracle_on_sample"] = same == cpu_events
+    out["surrogate"] = surrogate_section(L, stations, (nz, nx, ny, h), nev,
+                                         args, opt, tgpu)
     L.locate_setReferenceCompatibility(1, byref(ierr))
     L.locate_finalize()
     return out
+
+
+def surrogate_section(L, stations, grid, nev, args, full_opt, full_s):
+    """Config 5: MLP 3 -> 256^4 -> 48 trained on 2M (cell, times) pairs from
+    the locator's tables (seed 13), then Gauss-Newton proposals scored with
+    the exact tables; agreement and events/s against the full search."""
+    nz, nx, ny, h = grid
+    ierr = c_int(1)
+    dp = ctypes.POINTER(c_double)
+    ip = ctypes.POINTER(c_int)
+    nsta = len(stations)
+    sz = np.array([s[0] for s in stations])
+    sx = np.array([s[1] for s in stations])
+    sy = np.array([s[2] for s in stations])
+    L.fteik_surrogate_initialize(nsta, nz, nx, ny, h, h, h,
+                                 sz.ctypes.data_as(dp), sx.ctypes.data_as(dp),
+                                 sy.ctypes.data_as(dp), 6000.0, 1.0, 13,
+                                 byref(ierr))
+    assert ierr.value == 0
+    loss = c_double()
+    hip_sync()
+    t0 = time.perf_counter()
+    L.fteik_surrogate_train(args.sur_steps, 8192, 2e-3, 2000000, byref(loss),
+                            byref(ierr))
+    hip_sync()
+    ttrain = time.perf_counter() - t0
+    assert ierr.value == 0
+    ev = np.arange(1, nev + 1, dtype=np.int32)
+    opt = np.zeros(nev, dtype=np.int32)
+    t0o = np.zeros(nev)
+    obj = np.zeros(nev)
+    nfb = c_int()
+    call = (nev, ev.ctypes.data_as(ip), 8, 6, opt.ctypes.data_as(ip),
+            t0o.ctypes.data_as(dp), obj.ctypes.data_as(dp), byref(nfb),
+            byref(ierr))
+    L.locate_locateEventsSurrogate(*call)            # warm-up
+    hip_sync()
+    t0 = time.perf_counter()
+    L.locate_locateEventsSurrogate(*call)
+    hip_sync()
+    tsur = time.perf_counter() - t0
+    assert ierr.value == 0
+    L.fteik_surrogate_free()
+    return {"workload": "MLP 3-256-256-256-256-48 SiLU, BF16 MFMA training "
+                        "on 2M table samples, 8 Gauss-Newton starts x 6 "
+                        "steps, exact 4x4x4 cube scoring",
+            "train_steps": args.sur_steps, "train_batch": 8192,
+            "train_seconds": ttrain, "final_loss": loss.value,
+            "value": nev / tsur, "unit": "events/s", "seconds": tsur,
+            "speedup_vs_full_search": full_s / tsur,
+            "agreement_with_full_search": float(np.mean(opt == full_opt)),
+            "fallback_events": nfb.value}
 
 
 def hip_sync():
@@ -236,6 +295,8 @@ def main():
                     help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched-solve and locator sections")
+    ap.add_argument("--sur-steps", type=int, default=3000,
+                    help="training steps of the locator surrogate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
@@ -303,7 +364,7 @@ def main():
     # the workload uses the smallest count after which one more iteration
     # changes no node.
     prev, nsweep = None, None
-    for ns in range(1, 13):
+    for ns in range(1, 41):
         init(0, ns)
         run(1)
         cur = field()
@@ -312,7 +373,7 @@ def main():
             break
         prev = cur
     if nsweep is None:
-        nsweep = 12
+        nsweep = 40
     print("bench: FP64 converges after %d iterations" % nsweep,
           file=sys.stderr, flush=True)
 
@@ -337,6 +398,9 @@ def main():
     ms_launch32 = sweep_ms(1)
 
     units = float((N - 1) ** 3)          # updates per Gauss-Seidel launch
+    t1, t2, nt1 = c_int(), c_int(), c_int()
+    L.fteik_solver3d_getTile(byref(t1), byref(t2), byref(nt1), byref(ierr))
+    tile64 = t1.value
     ach64 = BYTES_PER_UPDATE["f64"] * units / (ms_launch64 * 1e-3) / 1e9
     traffic = None
     if os.path.isfile(TRAFFIC_FILE):
@@ -387,7 +451,7 @@ def main():
                   "max_rel_err_vs_fp64": float(rel.max()),
                   "median_rel_err_vs_fp64": float(np.median(rel)),
                   "fp32_sweep_ms_per_launch": ms_launch32},
-        "roofline": {"kernel": "sweep3d_pencil_kernel<double,false>",
+        "roofline": {"kernel": "FP64 Gauss-Seidel sweep, tile %d" % tile64,
                      "bound": "hbm", "achieved": ach64, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
                      "traffic": traffic,
@@ -403,7 +467,7 @@ def main():
         out["batch"] = batch_section(L, nt.value)
         print("bench: batch %.1f Gupd/s" % out["batch"]["value"],
               file=sys.stderr, flush=True)
-        out["locator"] = locator_section(L)
+        out["locator"] = locator_section(L, args)
         print("bench: locator %.0f events/s" % out["locator"]["value"],
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)

@@ -48,6 +48,7 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
     a.err = err;
     a.serial = serial;
     a.maxdelta = track ? g.maxdelta : nullptr;
+    a.mb = nullptr;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) {
         if (threadIdx.x == 0)
             __hip_atomic_store(flags + m.flag, (serial << 32) | 0xffffffffull,
@@ -68,7 +69,7 @@ static const void* batch_fn_t(int tile)
     }
 #undef BF
 }
-static_assert(kNumTiles == 7, "batch_fn_t lists every tile");
+// the batch uses the flag-exchange tiles only (kTiles[0..6])
 
 } // namespace fteik
 
@@ -119,7 +120,8 @@ extern "C" int fteik_k_batch_sweep(const BatchGrid* grids, const int* map,
                                    unsigned long long* flags, int* err,
                                    int fp32, int track, hipStream_t stream)
 {
-    if (tile < 0 || tile >= kNumTiles || nwg < 1) return 2;
+    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode != 0 || nwg < 1)
+        return 2;
     const TileShape ts = kTiles[tile];
     const void* fn = fp32 ? (linit ? batch_fn_t<float, true>(tile)
                                    : batch_fn_t<float, false>(tile))

@@ -242,8 +242,8 @@ struct Solver3D {
     int precision = 0, schedule = 0, cooperative = 1;
     int tile = 0, tile32 = 0;            // pipelined-sweep tile shapes
     std::vector<char> tileOK;            // tile fits co-resident (both types)
-    double mixedBias = 1.0/262144.0;     // 2^-18 relative promotion margin
-    int maxCorr = 16;
+    double mixedBias = 1.0/256.0;        // 2^-8 relative promotion margin
+    int maxCorr = 64;
     std::vector<SrcInfo> src;
     std::vector<int> ri;
     std::vector<double> rd;
@@ -253,6 +253,7 @@ struct Solver3D {
     float *d_slow32 = nullptr, *d_tt32 = nullptr;
     void *d_cst = nullptr, *d_cst32 = nullptr;
     unsigned long long *d_flags = nullptr;
+    unsigned long long *d_mb = nullptr;  // tagged mailbox, 2 words per node
     int *d_err = nullptr;
     double *d_maxdelta = nullptr;
     int *d_ri = nullptr;
@@ -272,6 +273,7 @@ void free3d()
     dfree(g3.d_slow32); dfree(g3.d_tt32);
     dfree(g3.d_cst); dfree(g3.d_cst32);
     dfree(g3.d_flags); dfree(g3.d_err); dfree(g3.d_maxdelta);
+    dfree(g3.d_mb);
     dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr);
     if (g3.ev0) hipEventDestroy(g3.ev0);
     if (g3.ev1) hipEventDestroy(g3.ev1);
@@ -306,6 +308,7 @@ SweepLaunch launch3d(bool fp32)
     L.schedule = g3.schedule;
     L.cooperative = g3.cooperative;
     L.tile = fp32 ? g3.tile32 : g3.tile;
+    L.mb = g3.d_mb;
     return L;
 }
 
@@ -538,6 +541,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
               dalloc((char **) &g3.d_cst, 512) &&
               dalloc((char **) &g3.d_cst32, 512) &&
               dalloc(&g3.d_flags, g3.max_flags) && dalloc(&g3.d_err, 1) &&
+              dalloc(&g3.d_mb, 2*(size_t) ngrd) &&
               dalloc(&g3.d_maxdelta, 1);
     if (!ok) {
         msg("fteik_solver3d_initialize: out of GPU memory");
@@ -548,18 +552,23 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     hipMemsetAsync(g3.d_flags, 0, g3.max_flags*sizeof(unsigned long long),
                    g_dev.stream);
     hipMemsetAsync(g3.d_err, 0, sizeof(int), g_dev.stream);
+    hipMemsetAsync(g3.d_mb, 0, 2*(size_t) ngrd*sizeof(unsigned long long),
+                   g_dev.stream);
     hipEventCreate(&g3.ev0);
     hipEventCreate(&g3.ev1);
     // tile shapes whose whole grid of workgroups can be co-resident (FP64
     // and FP32 kernels); the first such one is the default
     const int ntile = fteik_k_num_tiles();
     g3.tileOK.assign((size_t) ntile, 0);
-    int first = -1;
-    for (int t = 0; t < ntile; t++) {
+    for (int t = 0; t < ntile; t++)
         g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
                                         fteik_k_pencil_fits(nx, ny, 1, t));
-        if (g3.tileOK[(size_t) t] && first < 0) first = t;
-    }
+    // default: the first available shape in order of measured speed
+    // (257^3: 8x8 mailbox, 16x8 mailbox, 16x8 flags, 8x8 flags, ...)
+    static const int pref[] = {9, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
+    int first = -1;
+    for (int t : pref)
+        if (t < ntile && g3.tileOK[(size_t) t]) { first = t; break; }
     g3.cooperative = first >= 0;
     g3.tile = g3.tile32 = first >= 0 ? first : 0;
     if (!g3.cooperative) g3.schedule = 1;
@@ -2284,7 +2293,7 @@ struct BGrid {
 struct Batch3D {
     bool init = false, haveTimes = false;
     int precision = 0, tile = 6, maxCorr = 16;
-    double bias = 1.0/262144.0, convTol = 0.0;
+    double bias = 1.0/256.0, convTol = 0.0;
     std::vector<BGrid> g;
     fteik::BatchGrid *d_desc = nullptr;
     int *d_map = nullptr, nwg = 0, mapTile = -1;
@@ -2481,8 +2490,10 @@ void fteik_batch3d_setPrecision(int mode, int *ierr)
 void fteik_batch3d_setTile(int tile, int *ierr)
 {
     *ierr = 0;
-    if (tile < 0 || tile >= fteik_k_num_tiles()) {
-        msg("fteik_batch3d_setTile: tile %d out of range", tile);
+    if (tile < 0 || tile >= fteik_k_num_tiles() ||
+        fteik_k_tile_mode(tile) != 0) {
+        msg("fteik_batch3d_setTile: tile %d is not available for batches",
+            tile);
         *ierr = 1;
         return;
     }

@@ -28,6 +28,7 @@ struct SweepLaunch {
     int schedule;                // 0: pipelined column blocks, 1: per level
     int cooperative;
     int tile;                    // tile shape index of the pipelined sweep
+    unsigned long long* mb;      // mailbox of the mode-1 tiles
 };
 
 // One grid of a batched solve (batch3d.hip); device pointers.
@@ -53,6 +54,7 @@ int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
                     hipStream_t stream);
 int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile);
 int fteik_k_num_tiles(void);
+int fteik_k_tile_mode(int tile);
 int fteik_k_max_flags(int nx, int ny);
 int fteik_k_source3d_init(void* cst, void* tt, const void* slow,
                           const fteik::SourceGeom* g, long ngrd, int fp32,

@@ -34,8 +34,14 @@ struct Consts {
     int zsi, xsi, ysi, pad;
 };
 
-template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return (a < b) ? a : b; }
-template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return (a > b) ? a : b; }
+// Minimum and maximum of travel times and slownesses.  The operands are
+// never NaN (every square root is guarded) and never -0, so the hardware
+// min/max (one v_min_f64/v_max_f64 instead of a compare and two selects)
+// returns exactly what the reference's MIN/MAX return.
+__device__ __forceinline__ double rmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double rmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float rmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double rsqrt_exact(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float rsqrt_exact(float x) { return __builtin_sqrtf(x); }
 

@@ -419,33 +419,16 @@ __global__ void k_inv_gn(const float* __restrict__ y, long R, int ldy,
 
 // Exact scoring of the 4 x 4 x 4 cells around each event's best start, one
 // wave per event (one lane per cell), with the locator's objective in its
-// operation order; flags events whose minimum lies on the cube's faces.
-__global__ void k_inv_score(const float* __restrict__ pos,
-                            const float* __restrict__ misfit, int starts,
-                            int nev, GeoArgs g, const float* __restrict__ tab,
-                            long ldgrd, const float* __restrict__ obs,
-                            const float* __restrict__ wt,
-                            const unsigned char* __restrict__ skip,
-                            const float* __restrict__ xnorm, int lntf,
-                            unsigned long long* __restrict__ best,
-                            int* __restrict__ fallback)
+// operation order.  If the best cell lies on a face of the cube that is not
+// a face of the grid, the cube is re-centred on it and scored again (at
+// most `rounds` times); events still on a face are flagged for the full
+// grid search.
+__device__ __forceinline__ unsigned long long cube_min(
+    int i0, int j0, int k0, int lane, int e, const GeoArgs& g,
+    const float* __restrict__ tab, long ldgrd, const float* __restrict__ obs,
+    const float* __restrict__ wt, const unsigned char* __restrict__ skip,
+    const float* __restrict__ xnorm, int lntf)
 {
-    const int e = blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (e >= nev) return;
-    int bs = 0;
-    float bf = misfit[(long) e*starts];
-    for (int s = 1; s < starts; s++) {
-        const float f = misfit[(long) e*starts + s];
-        if (f < bf) { bf = f; bs = s; }
-    }
-    const float* p = pos + 3*((long) e*starts + bs);
-    const int cj = (int) rintf((p[0] + 1.0f)*0.5f*(float) (g.nx - 1));
-    const int ck = (int) rintf((p[1] + 1.0f)*0.5f*(float) (g.ny - 1));
-    const int ci = (int) rintf((p[2] + 1.0f)*0.5f*(float) (g.nz - 1));
-    const int i0 = min(max(ci - 1, 0), g.nz - 4);
-    const int j0 = min(max(cj - 1, 0), g.nx - 4);
-    const int k0 = min(max(ck - 1, 0), g.ny - 4);
     const int di = lane & 3, dj = (lane >> 2) & 3, dk = lane >> 4;
     const long c = (long) (k0 + dk)*g.nz*g.nx + (long) (j0 + dj)*g.nz + i0 + di;
     const float* o = obs + (long) e*lntf;
@@ -472,15 +455,49 @@ __global__ void k_inv_score(const float* __restrict__ pos,
         const unsigned long long q = __shfl_xor(key, off, 64);
         key = q < key ? q : key;
     }
-    // face test: is the winning cell on a face of the cube that is not
-    // also a face of the grid?
-    const long wc = (long) (key & 0xffffffffull);
-    const long nzx = (long) g.nz*g.nx;
-    const int wk = (int) (wc / nzx), wj = (int) ((wc % nzx)/g.nz),
-              wi = (int) (wc % g.nz);
-    const bool face = (wi == i0 && i0 > 0) || (wi == i0 + 3 && i0 + 4 < g.nz)
-                   || (wj == j0 && j0 > 0) || (wj == j0 + 3 && j0 + 4 < g.nx)
-                   || (wk == k0 && k0 > 0) || (wk == k0 + 3 && k0 + 4 < g.ny);
+    return key;
+}
+
+__global__ void k_inv_score(const float* __restrict__ pos,
+                            const float* __restrict__ misfit, int starts,
+                            int nev, GeoArgs g, const float* __restrict__ tab,
+                            long ldgrd, const float* __restrict__ obs,
+                            const float* __restrict__ wt,
+                            const unsigned char* __restrict__ skip,
+                            const float* __restrict__ xnorm, int lntf,
+                            int rounds, unsigned long long* __restrict__ best,
+                            int* __restrict__ fallback)
+{
+    const int e = blockIdx.x*(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e >= nev) return;
+    int bs = 0;
+    float bf = misfit[(long) e*starts];
+    for (int s = 1; s < starts; s++) {
+        const float f = misfit[(long) e*starts + s];
+        if (f < bf) { bf = f; bs = s; }
+    }
+    const float* p = pos + 3*((long) e*starts + bs);
+    int cj = (int) rintf((p[0] + 1.0f)*0.5f*(float) (g.nx - 1));
+    int ck = (int) rintf((p[1] + 1.0f)*0.5f*(float) (g.ny - 1));
+    int ci = (int) rintf((p[2] + 1.0f)*0.5f*(float) (g.nz - 1));
+    unsigned long long key = ~0ull;
+    bool face = true;
+    for (int r = 0; r < rounds && face; r++) {
+        const int i0 = min(max(ci - 1, 0), g.nz - 4);
+        const int j0 = min(max(cj - 1, 0), g.nx - 4);
+        const int k0 = min(max(ck - 1, 0), g.ny - 4);
+        key = cube_min(i0, j0, k0, lane, e, g, tab, ldgrd, obs, wt, skip,
+                       xnorm, lntf);
+        const long wc = (long) (key & 0xffffffffull);
+        const long nzx = (long) g.nz*g.nx;
+        const int wk = (int) (wc / nzx), wj = (int) ((wc % nzx)/g.nz),
+                  wi = (int) (wc % g.nz);
+        face = (wi == i0 && i0 > 0) || (wi == i0 + 3 && i0 + 4 < g.nz)
+            || (wj == j0 && j0 > 0) || (wj == j0 + 3 && j0 + 4 < g.nx)
+            || (wk == k0 && k0 > 0) || (wk == k0 + 3 && k0 + 4 < g.ny);
+        ci = wi; cj = wj; ck = wk;
+    }
     if (lane == 0) {
         best[e] = key;
         fallback[e] = face ? 1 : 0;
@@ -642,7 +659,7 @@ int fteik_k_inv_score(const float* pos, const float* misfit, int starts,
     hipLaunchKernelGGL(k_inv_score, dim3((nev + wpb - 1)/wpb), dim3(64*wpb),
                        0, s, pos, misfit, starts, nev,
                        *(const GeoArgs*) geo, tab, ldgrd, obs, wt, skip, xnorm,
-                       lntf, best, fallback);
+                       lntf, 4, best, fallback);
     return ok();
 }
 

@@ -92,6 +92,15 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
     pencil_body<R, INIT, BX, BY, HB>(a, blockIdx.x);
 }
 
+// at least 3 waves per SIMD, so two workgroups (5 waves each) share a CU
+template <typename R, bool INIT, int BX, int BY, int HB>
+__global__ __launch_bounds__(BX*BY + 64)
+__attribute__((amdgpu_waves_per_eu(3)))
+void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
+{
+    pencil_mb_body<R, INIT, BX, BY, HB>(a, blockIdx.x);
+}
+
 // Simple reference schedule: one launch per level (used for cross-checks
 // and as a fallback for grids whose column blocks cannot be co-resident).
 template <typename R, bool INIT>
@@ -138,13 +147,23 @@ static const void* pencil_fn_t(int tile)
 {
 #define PF(n) case n: return (const void*) sweep3d_pencil_kernel<R, INIT, \
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
+#define PM(n) case n: return (const void*) sweep3d_pencil_mb_kernel<R, INIT, \
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
     switch (tile) {
         PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
+        PM(7) PM(8) PM(9) PM(10)
         default: return nullptr;
     }
 #undef PF
+#undef PM
 }
-static_assert(kNumTiles == 7, "pencil_fn_t lists every tile");
+static_assert(kNumTiles == 11 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
+              "pencil_fn_t lists every tile");
+
+static int tile_threads(int tile)
+{
+    return kTiles[tile].bx*kTiles[tile].by + (kTiles[tile].mode ? 64 : 0);
+}
 
 template <typename R>
 static const void* pencil_fn(bool linit, int tile)
@@ -167,6 +186,7 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
     a.fy0 = b[4]; a.fy1 = b[5];
     a.maxdelta = L.maxdelta;
+    a.mb = L.mb;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) return 0;
     if (L.schedule == 1) {
         const int nlev = (a.fz1 + a.fx1 + a.fy1) - (a.fz0 + a.fx0 + a.fy0) + 1;
@@ -192,11 +212,12 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     void* args[] = {&a};
     hipError_t e;
     const void* fn = pencil_fn<R>(linit, tile);
+    if (ts.mode == 1 && !L.mb) return 2;
     if (L.cooperative)
-        e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(ts.bx*ts.by), args,
-                                       0, stream);
+        e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(tile_threads(tile)),
+                                       args, 0, stream);
     else
-        e = hipLaunchKernel(fn, dim3(nblk), dim3(ts.bx*ts.by), args, 0,
+        e = hipLaunchKernel(fn, dim3(nblk), dim3(tile_threads(tile)), args, 0,
                             stream);
     return e == hipSuccess ? 0 : 1;
 }
@@ -227,7 +248,7 @@ extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
                               : pencil_fn<double>(v == 1, tile);
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
-                                                         ts.bx*ts.by, 0)
+                                                         tile_threads(tile), 0)
             != hipSuccess) return 0;
         if ((long) per_cu*ncu < nblk) return 0;
     }
@@ -235,6 +256,11 @@ extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
 }
 
 extern "C" int fteik_k_num_tiles(void) { return kNumTiles; }
+
+extern "C" int fteik_k_tile_mode(int tile)
+{
+    return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].mode : -1;
+}
 
 extern "C" int fteik_k_max_flags(int nx, int ny)
 {

@@ -30,10 +30,15 @@ namespace fteik {
 // Tile shapes of the pipelined sweep: BX x BY columns per workgroup, halo
 // exchanged with the upwind workgroups every HB steps.  The autotuner picks
 // one per grid (fteik_solver3d_autotune); kTiles[0] is the default.
-struct TileShape { int bx, by, hb; };
+// mode 0: upwind tiles publish a progress flag every hb steps and the
+//         downwind tile pulls its halo after the flag (pencil_body);
+// mode 1: boundary values go to a tagged mailbox and a dedicated halo wave
+//         polls it ahead of the compute waves (pencil_mb_body).
+struct TileShape { int bx, by, hb, mode; };
 constexpr TileShape kTiles[] = {
-    {16, 16, 8}, {16, 16, 4}, {16, 16, 2}, {8, 8, 4}, {8, 8, 2}, {32, 8, 4},
-    {16, 8, 4},
+    {16, 16, 8, 0}, {16, 16, 4, 0}, {16, 16, 2, 0}, {8, 8, 4, 0},
+    {8, 8, 2, 0}, {32, 8, 4, 0}, {16, 8, 4, 0},
+    {16, 16, 4, 1}, {16, 8, 4, 1}, {8, 8, 4, 1}, {32, 8, 4, 1},
 };
 constexpr int kNumTiles = (int) (sizeof(kTiles)/sizeof(kTiles[0]));
 constexpr unsigned long long kSpinLimit = 1ull << 22;
@@ -78,6 +83,7 @@ struct SweepArgs {
     int fz0, fz1, fx0, fx1, fy0, fy1;   // update box, flipped 0-based
     int nbx, nby;                // blocks along flipped x, y
     double* maxdelta;            // optional: max |t_new - t_old| (as bits)
+    unsigned long long* mb;      // mailbox (mode 1): 2 words/node FP64, 1 FP32
 };
 
 template <typename R>
@@ -353,6 +359,248 @@ __device__ __forceinline__ void pencil_body(const SweepArgs<R>& a, const int wg)
                            __HIP_MEMORY_SCOPE_AGENT);
     if (a.maxdelta && dmax > R(0)) {
         // positive doubles order like their bit patterns
+        double dd = (double) dmax;
+        atomicMax((unsigned long long*) a.maxdelta,
+                  (unsigned long long) __double_as_longlong(dd));
+    }
+}
+
+// ------------------------------------------------------------ mailbox
+// A node value tagged with the launch serial in each 64-bit word, so a
+// reader can tell this sweep's value from an older one without any
+// ordering between separate stores: FP64 uses two words (low and high
+// halves), FP32 one.
+__device__ __forceinline__ void mb_put(unsigned long long* mb, long g,
+                                       double v, unsigned tag)
+{
+    const unsigned long long b = (unsigned long long) __double_as_longlong(v);
+    const unsigned long long t = (unsigned long long) tag << 32;
+    __hip_atomic_store(mb + 2*g, t | (b & 0xffffffffull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(mb + 2*g + 1, t | (b >> 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void mb_put(unsigned long long* mb, long g,
+                                       float v, unsigned tag)
+{
+    __hip_atomic_store(mb + g, ((unsigned long long) tag << 32)
+                               | (unsigned long long) __float_as_uint(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool mb_get(unsigned long long* mb, long g,
+                                       unsigned tag, double& v)
+{
+    const unsigned long long lo = __hip_atomic_load(mb + 2*g, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi = __hip_atomic_load(mb + 2*g + 1,
+                                                    __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned) (lo >> 32) != tag || (unsigned) (hi >> 32) != tag)
+        return false;
+    v = __longlong_as_double((long long) ((hi << 32) | (lo & 0xffffffffull)));
+    return true;
+}
+__device__ __forceinline__ bool mb_get(unsigned long long* mb, long g,
+                                       unsigned tag, float& v)
+{
+    const unsigned long long w = __hip_atomic_load(mb + g, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned) (w >> 32) != tag) return false;
+    v = __uint_as_float((unsigned) (w & 0xffffffffull));
+    return true;
+}
+
+__device__ __forceinline__ int lds_load(int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Mode-1 sweep of one tile: BX*BY compute threads (one column each) plus
+// one halo wave.  The halo wave polls the mailbox for the upwind columns'
+// values of the coming steps and writes them into the LDS ring; compute
+// waves wait on two LDS counters (halo steps ready, compute wave-steps
+// done) instead of workgroup barriers, and boundary columns post their
+// values to the mailbox as they compute them.  No progress flags and no
+// wait for store completion are needed: every mailbox word carries the
+// launch serial.
+template <typename R, bool INIT, int BX, int BY, int HB>
+__device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
+                                               const int wg)
+{
+    constexpr int NC = BX*BY;
+    constexpr int NW = NC/64;
+    constexpr int RING = 16;
+    constexpr int HCOLS = BX + BY + 1;
+    static_assert(NC % 64 == 0 && HCOLS <= 64 && HB + 3 <= RING, "tile");
+    __shared__ R ring[RING][(BX + 1)*(BY + 1)];
+    __shared__ int s_cdone, s_hready, s_abort;
+    const int fbx = wg / a.nby, fby = wg - (wg / a.nby)*a.nby;
+    const int fj0 = fbx*BX, fk0 = fby*BY;
+    const int nz = a.nz, nx = a.nx, ny = a.ny;
+    const int ez0 = a.fz0 - 1, ex0 = a.fx0 - 1, ey0 = a.fy0 - 1;
+    const int jlo = max(fj0, ex0), jhi = min(min(fj0 + BX - 1, a.fx1), nx - 1);
+    const int klo = max(fk0, ey0), khi = min(min(fk0 + BY - 1, a.fy1), ny - 1);
+    // a tile outside the extended box owns no node anyone reads
+    if (jlo > jhi || klo > khi) return;
+    const int tlo = ez0 + jlo + klo;
+    const int thi = a.fz1 + jhi + khi;
+    const unsigned tag = (unsigned) a.serial;
+    if (threadIdx.x == 0) { s_cdone = 0; s_hready = tlo - 3; s_abort = 0; }
+    __syncthreads();
+
+    if (threadIdx.x >= NC) {
+        // ---------------------------------------------------- halo wave
+        const int hl = threadIdx.x - NC;
+        int hj = 0, hk = 0, slot = 0;
+        if (hl < BY)            { hj = fj0 - 1; hk = fk0 + hl;
+                                  slot = (hl + 1)*(BX + 1); }
+        else if (hl < BY + BX)  { hj = fj0 + (hl - BY); hk = fk0 - 1;
+                                  slot = hl - BY + 1; }
+        else                    { hj = fj0 - 1; hk = fk0 - 1; slot = 0; }
+        // the producer writes exactly its extended-box columns
+        const bool hcol = hl < HCOLS && hj >= ex0 && hk >= ey0 &&
+                          hj <= min(a.fx1, nx - 1) && hk <= min(a.fy1, ny - 1);
+        const long gcol = hcol ? (long) (unflip(hk, ny, a.sy) - 1)*nz*nx
+                               + (long) (unflip(hj, nx, a.sx) - 1)*nz : 0;
+        for (int s = tlo - 3; s <= thi - 1; s++) {
+            // the ring slot of step s was last read at step s - RING + 3
+            const int need = NW*(s - RING + 3 - tlo + 1);
+            unsigned long long spins = 0;
+            while (lds_load(&s_cdone) < need) {
+                if (++spins > kSpinLimit || lds_load(&s_abort)) return;
+            }
+            const int fi = s - hj - hk;
+            if (hcol && fi >= ez0 && fi <= a.fz1) {
+                const long g = gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
+                R v;
+                spins = 0;
+                while (!mb_get(a.mb, g, tag, v)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSpinLimit) {
+                        atomicExch(a.err, 1);
+                        __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                }
+                ring[s & (RING - 1)][slot] = v;
+            }
+            if (lds_load(&s_abort)) return;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            if (hl == 0)
+                __hip_atomic_store(&s_hready, s + 1, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        return;
+    }
+
+    // -------------------------------------------------------- compute waves
+    const int lj = threadIdx.x % BX, lk = threadIdx.x / BX;
+    const int fj = fj0 + lj, fk = fk0 + lk;
+    const bool colE = (fj >= jlo && fj <= jhi && fk >= klo && fk <= khi);
+    const bool colB = colE && fj >= a.fx0 && fk >= a.fy0;
+    const bool post = colE && (lj == BX - 1 || lk == BY - 1);
+    const int cj = min(max(fj, jlo), jhi), ck = min(max(fk, klo), khi);
+    const int j = unflip(cj, nx, a.sx), k = unflip(ck, ny, a.sy);
+    const long colbase = (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
+    R* const col = a.tt + colbase;
+    const int jm = max(j - 1, 1), jp = min(j, nx - 1);
+    const int km = max(k - 1, 1), kp = min(k, ny - 1);
+    const int xu = (a.sx > 0) ? jm : jp, xd = (a.sx > 0) ? jp : jm;
+    const int yu = (a.sy > 0) ? km : kp, yd = (a.sy > 0) ? kp : km;
+    const long szx = (long) (nz - 1)*(nx - 1);
+    const R* const c_uu = a.slow + (long) (yu - 1)*szx + (long) (xu - 1)*(nz - 1);
+    const R* const c_ud = a.slow + (long) (yd - 1)*szx + (long) (xu - 1)*(nz - 1);
+    const R* const c_du = a.slow + (long) (yu - 1)*szx + (long) (xd - 1)*(nz - 1);
+    const R* const c_dd = a.slow + (long) (yd - 1)*szx + (long) (xd - 1)*(nz - 1);
+    const int fdep = cj + ck;
+    const Consts<R> c = *a.cst;
+    const int me = (lk + 1)*(BX + 1) + (lj + 1);
+    const int cW = me - 1, cS = me - (BX + 1), cSW = me - (BX + 2);
+    R dmax = R(0);
+    auto node_off = [&](int fi) -> int {
+        fi = min(max(fi, ez0), a.fz1);
+        return (a.sz > 0) ? fi : nz - 1 - fi;
+    };
+    auto layer_off = [&](int f) -> int {
+        const int zc = (a.sz > 0) ? f : nz - f;
+        return min(max(zc, 1), nz - 1) - 1;
+    };
+    R tn[HB], cn[HB + 1][4];
+    auto prefetch = [&](int tb) {
+        const int f0 = tb - fdep;
+        #pragma unroll
+        for (int q = 0; q < HB; q++) tn[q] = col[node_off(f0 + q)];
+        #pragma unroll
+        for (int q = 0; q <= HB; q++) {
+            const int z = layer_off(f0 + q);
+            cn[q][0] = c_uu[z]; cn[q][1] = c_ud[z];
+            cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
+        }
+    };
+    if (!INIT) prefetch(tlo);
+    for (int tb = tlo; tb <= thi; tb += HB) {
+        if (INIT) prefetch(tb);
+        R tc[HB], cc[HB + 1][4];
+        #pragma unroll
+        for (int q = 0; q < HB; q++) tc[q] = tn[q];
+        #pragma unroll
+        for (int q = 0; q <= HB; q++) {
+            cc[q][0] = cn[q][0]; cc[q][1] = cn[q][1];
+            cc[q][2] = cn[q][2]; cc[q][3] = cn[q][3];
+        }
+        if (!INIT && tb + HB <= thi) prefetch(tb + HB);
+        #pragma unroll
+        for (int q = 0; q < HB; q++) {
+            const int t = tb + q;
+            if (t > thi) break;                       // workgroup-uniform
+            // halo of steps < t and every compute wave past step t-1
+            const int needC = NW*(t - tlo);
+            unsigned long long spins = 0;
+            while (lds_load(&s_hready) < t || lds_load(&s_cdone) < needC) {
+                if (lds_load(&s_abort) || ++spins > kSpinLimit) {
+                    atomicExch(a.err, 1);
+                    return;
+                }
+            }
+            const int fi = t - fj - fk;
+            if (colE && fi >= ez0 && fi <= a.fz1) {
+                R v = tc[q];
+                if (colB && fi >= a.fz0) {
+                    const int s1 = (t - 1) & (RING - 1),
+                              s2 = (t - 2) & (RING - 1),
+                              s3 = (t - 3) & (RING - 1);
+                    Nbr<R> nb;
+                    nb.tv   = ring[s1][me];
+                    nb.te   = ring[s1][cW];
+                    nb.tn   = ring[s1][cS];
+                    nb.tev  = ring[s2][cW];
+                    nb.ten  = ring[s2][cSW];
+                    nb.tnv  = ring[s2][cS];
+                    nb.tnve = ring[s3][cSW];
+                    Slw<R> sl;
+                    slowness7_ud<R>(cc[q], cc[q + 1], sl);
+                    R vn;
+                    if (INIT) {
+                        const int i = unflip(fi, nz, a.sz);
+                        vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
+                    } else {
+                        vn = noinit3d(c, nb, sl, v);
+                    }
+                    dmax = rmax(dmax, v - vn);
+                    v = vn;
+                    col[node_off(fi)] = v;
+                }
+                if (post) mb_put(a.mb, colbase + node_off(fi), v, tag);
+                ring[t & (RING - 1)][me] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            if ((threadIdx.x & 63) == 0)
+                __hip_atomic_fetch_add(&s_cdone, 1, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    if (a.maxdelta && dmax > R(0)) {
         double dd = (double) dmax;
         atomicMax((unsigned long long*) a.maxdelta,
                   (unsigned long long) __double_as_longlong(dd));

@@ -94,9 +94,9 @@ void fteik_solver3d_setPrecision(int mode, int *ierr);
  * 1 = one kernel launch per level (diagnostic schedule). */
 void fteik_solver3d_setSchedule(int schedule, int *ierr);
 /* EXTENSION: mixed-mode parameters: the single-precision field is raised
- * by the relative margin bias (default 2^-18) when it is promoted, so it
+ * by the relative margin bias (default 2^-8) when it is promoted, so it
  * stays an upper bound of the double-precision solution and the double
- * correction iterations (at most maxCorrections, default 16) converge onto
+ * correction iterations (at most maxCorrections, default 64) converge onto
  * it.  Reset to the defaults by fteik_solver3d_initialize. */
 void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
                                        int *ierr);
@@ -250,8 +250,9 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
                            double *lossOut, int *ierr);
 /* surrogate-pruned location (fresh origin-time state): Gauss-Newton on the
  * surrogate's misfit from `starts` (<= 9) points, exact scoring of the
- * 4 x 4 x 4 cells around the best, full grid search for events whose best
- * cell lies on the cube's faces (counted in nFallback). */
+ * 4 x 4 x 4 cells around the best (re-centred on the best cell up to 4
+ * times while it lies on a face of the cube), full grid search for events
+ * whose best cell still lies on a face (counted in nFallback). */
 void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
                                   int iterations, int *optIndx, double *t0Opt,
                                   double *objOpt, int *nFallback, int *ierr);
