@@ -87,6 +87,12 @@ template <typename R>
 struct Slw { R s1, s2, s3, s4, s5, s6, s7; };
 
 // localSolver.F90:384-534 for one node; tupd_in is the node's current time.
+// Every operator is evaluated unconditionally and the reference's gates
+// only select its value (or HUGE): the four square roots are independent,
+// so the compiler can interleave them instead of branching around each.
+// Where a gate is closed the value computed under it is discarded (it may
+// be NaN); where it is open the value is computed in the reference's
+// operation order, so results are bitwise those of the gated form.
 template <typename R>
 __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
                                       const Slw<R>& s, R tupd_in)
@@ -96,73 +102,75 @@ __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
     t12min = rmin(t12min, t.te + c.dx*s.s2);
     t12min = rmin(t12min, t.tn + c.dy*s.s3);
     // ZX plane
-    R t1_2d = huge;
-    R temtv = t.te - t.tv;
-    if (temtv < c.dz*s.s4 && -temtv < c.dx*s.s4) {
-        R sref2 = s.s4*s.s4;
-        R ta = t.tev + temtv;
-        R tb = t.tev - temtv;
-        R tab = ta - tb;
-        R tab2 = tab*tab;
-        R four_sref2 = R(4)*sref2;
+    const R temtv = t.te - t.tv;
+    R t1_2d;
+    {
+        const R sref2 = s.s4*s.s4;
+        const R ta = t.tev + temtv;
+        const R tb = t.tev - temtv;
+        const R tab = ta - tb;
+        const R tab2 = tab*tab;
+        const R four_sref2 = R(4)*sref2;
         t1_2d = ((tb*c.dz2i + ta*c.dx2i)
               + rsqrt_exact(four_sref2*c.dz2i_p_dx2i - c.dz2i_dx2i*tab2))
               *c.dz2i_p_dx2i_inv;
     }
-    t12min = rmin(t12min, t1_2d);
+    const bool g1 = temtv < c.dz*s.s4 && -temtv < c.dx*s.s4;
     // ZY plane
-    R t2_2d = huge;
-    R tvmtn = t.tv - t.tn;
-    if (tvmtn < c.dy*s.s5 && -tvmtn < c.dz*s.s5) {
-        R sref2 = s.s5*s.s5;
-        R ta = tvmtn + t.tnv;
-        R tb = -tvmtn + t.tnv;
-        R tab = ta - tb;
-        R tab2 = tab*tab;
-        R four_sref2 = R(4)*sref2;
+    const R tvmtn = t.tv - t.tn;
+    R t2_2d;
+    {
+        const R sref2 = s.s5*s.s5;
+        const R ta = tvmtn + t.tnv;
+        const R tb = -tvmtn + t.tnv;
+        const R tab = ta - tb;
+        const R tab2 = tab*tab;
+        const R four_sref2 = R(4)*sref2;
         t2_2d = ((ta*c.dz2i + tb*c.dy2i)
               + rsqrt_exact(four_sref2*c.dz2i_p_dy2i - c.dz2i_dy2i*tab2))
               *c.dz2i_p_dy2i_inv;
     }
-    t12min = rmin(t12min, t2_2d);
+    const bool g2 = tvmtn < c.dy*s.s5 && -tvmtn < c.dz*s.s5;
     // XY plane
-    R t3_2d = huge;
-    R temtn = t.te - t.tn;
-    if (temtn < c.dy*s.s6 && -temtn < c.dx*s.s6) {
-        R sref2 = s.s6*s.s6;
-        R ta = temtn + t.ten;
-        R tb = -temtn + t.ten;
-        R tab = ta - tb;
-        R tab2 = tab*tab;
-        R four_sref2 = R(4)*sref2;
+    const R temtn = t.te - t.tn;
+    R t3_2d;
+    {
+        const R sref2 = s.s6*s.s6;
+        const R ta = temtn + t.ten;
+        const R tb = -temtn + t.ten;
+        const R tab = ta - tb;
+        const R tab2 = tab*tab;
+        const R four_sref2 = R(4)*sref2;
         t3_2d = ((ta*c.dx2i + tb*c.dy2i)
               + rsqrt_exact(four_sref2*c.dx2i_p_dy2i - c.dx2i_dy2i*tab2))
               *c.dx2i_p_dy2i_inv;
     }
-    t12min = rmin(t12min, t3_2d);
+    const bool g3 = temtn < c.dy*s.s6 && -temtn < c.dx*s.s6;
     // 3D operator
-    R t3d = huge;
-    R tmax = rmax(rmax(t.tv, t.te), t.tn);
-    if (t12min > tmax) {
-        R sref2 = s.s7*s.s7;
-        R ta = t.te + R(0.5)*(-t.tn + t.ten - t.tv + t.tev) - t.tnv + t.tnve;
-        R tb = t.tv + R(0.5)*(-t.tn + t.tnv - t.te + t.tev) - t.ten + t.tnve;
-        R tc = t.tn + R(0.5)*(-t.te + t.ten - t.tv + t.tnv) - t.tev + t.tnve;
-        R tab = ta - tb;
-        R tbc = tb - tc;
-        R tac = ta - tc;
-        R t2 = sref2*c.dsum_nine;
-        R tab2 = tab*tab;
-        R tbc2 = tbc*tbc;
-        R tac2 = tac*tac;
-        R t3 = c.dz2dx2*tab2 + c.dz2dy2*tbc2 + c.dx2dy2*tac2;
-        if (t2 >= t3) {
-            R t1 = tb*c.dz2i + ta*c.dx2i + tc*c.dy2i;
-            t3d = (t1 + rsqrt_exact(t2 - t3))*c.dsumi;
-        }
+    R t3d_v, t2, t3;
+    {
+        const R sref2 = s.s7*s.s7;
+        const R ta = t.te + R(0.5)*(-t.tn + t.ten - t.tv + t.tev) - t.tnv + t.tnve;
+        const R tb = t.tv + R(0.5)*(-t.tn + t.tnv - t.te + t.tev) - t.ten + t.tnve;
+        const R tc = t.tn + R(0.5)*(-t.te + t.ten - t.tv + t.tnv) - t.tev + t.tnve;
+        const R tab = ta - tb;
+        const R tbc = tb - tc;
+        const R tac = ta - tc;
+        t2 = sref2*c.dsum_nine;
+        const R tab2 = tab*tab;
+        const R tbc2 = tbc*tbc;
+        const R tac2 = tac*tac;
+        t3 = c.dz2dx2*tab2 + c.dz2dy2*tbc2 + c.dx2dy2*tac2;
+        const R t1 = tb*c.dz2i + ta*c.dx2i + tc*c.dy2i;
+        t3d_v = (t1 + rsqrt_exact(t2 - t3))*c.dsumi;
     }
-    t3d = rmin(t3d, t12min);
-    return rmin(tupd_in, t3d);
+    // the reference's order of minimisation
+    t12min = rmin(t12min, g1 ? t1_2d : huge);
+    t12min = rmin(t12min, g2 ? t2_2d : huge);
+    t12min = rmin(t12min, g3 ? t3_2d : huge);
+    const R tmax = rmax(rmax(t.tv, t.te), t.tn);
+    const R t3d = (t12min > tmax && t2 >= t3) ? t3d_v : huge;
+    return rmin(tupd_in, rmin(t3d, t12min));
 }
 
 // localSolver.F90:538-751 for one node.  (i, j, k) are 1-based, sgnt* are
