@@ -563,9 +563,9 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     for (int t = 0; t < ntile; t++)
         g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
                                         fteik_k_pencil_fits(nx, ny, 1, t));
-    // default: the first available shape in order of measured speed
-    // (257^3: 8x8 mailbox, 16x8 mailbox, 16x8 flags, 8x8 flags, ...)
-    static const int pref[] = {9, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
+    // default: the first available shape in order of measured speed (8x8
+    // mailbox with store wave, 8x8 mailbox, 16x8 ..., then flag tiles)
+    static const int pref[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
     int first = -1;
     for (int t : pref)
         if (t < ntile && g3.tileOK[(size_t) t]) { first = t; break; }

@@ -92,13 +92,14 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
     pencil_body<R, INIT, BX, BY, HB>(a, blockIdx.x);
 }
 
-// at least 3 waves per SIMD, so two workgroups (5 waves each) share a CU
-template <typename R, bool INIT, int BX, int BY, int HB>
-__global__ __launch_bounds__(BX*BY + 64)
+// compute waves + halo wave (+ store wave in mode 2); at least 3 waves per
+// SIMD so two workgroups share a CU
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
+__global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(3)))
 void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
 {
-    pencil_mb_body<R, INIT, BX, BY, HB>(a, blockIdx.x);
+    pencil_mb_body<R, INIT, BX, BY, HB, MODE>(a, blockIdx.x);
 }
 
 // Simple reference schedule: one launch per level (used for cross-checks
@@ -148,21 +149,21 @@ static const void* pencil_fn_t(int tile)
 #define PF(n) case n: return (const void*) sweep3d_pencil_kernel<R, INIT, \
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
 #define PM(n) case n: return (const void*) sweep3d_pencil_mb_kernel<R, INIT, \
-        kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
     switch (tile) {
         PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
-        PM(7) PM(8) PM(9) PM(10)
+        PM(7) PM(8) PM(9) PM(10) PM(11) PM(12)
         default: return nullptr;
     }
 #undef PF
 #undef PM
 }
-static_assert(kNumTiles == 11 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
+static_assert(kNumTiles == 13 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
               "pencil_fn_t lists every tile");
 
 static int tile_threads(int tile)
 {
-    return kTiles[tile].bx*kTiles[tile].by + (kTiles[tile].mode ? 64 : 0);
+    return kTiles[tile].bx*kTiles[tile].by + 64*kTiles[tile].mode;
 }
 
 template <typename R>
@@ -212,7 +213,7 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     void* args[] = {&a};
     hipError_t e;
     const void* fn = pencil_fn<R>(linit, tile);
-    if (ts.mode == 1 && !L.mb) return 2;
+    if (ts.mode >= 1 && !L.mb) return 2;
     if (L.cooperative)
         e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(tile_threads(tile)),
                                        args, 0, stream);

@@ -33,12 +33,15 @@ namespace fteik {
 // mode 0: upwind tiles publish a progress flag every hb steps and the
 //         downwind tile pulls its halo after the flag (pencil_body);
 // mode 1: boundary values go to a tagged mailbox and a dedicated halo wave
-//         polls it ahead of the compute waves (pencil_mb_body).
+//         polls it ahead of the compute waves (pencil_mb_body);
+// mode 2: as mode 1, plus a store wave that takes the global stores off the
+//         compute waves.
 struct TileShape { int bx, by, hb, mode; };
 constexpr TileShape kTiles[] = {
     {16, 16, 8, 0}, {16, 16, 4, 0}, {16, 16, 2, 0}, {8, 8, 4, 0},
     {8, 8, 2, 0}, {32, 8, 4, 0}, {16, 8, 4, 0},
-    {16, 16, 4, 1}, {16, 8, 4, 1}, {8, 8, 4, 1}, {32, 8, 4, 1},
+    {16, 16, 4, 1}, {16, 8, 4, 1}, {8, 8, 4, 1},
+    {16, 16, 4, 2}, {16, 8, 4, 2}, {8, 8, 4, 2},
 };
 constexpr int kNumTiles = (int) (sizeof(kTiles)/sizeof(kTiles[0]));
 constexpr unsigned long long kSpinLimit = 1ull << 22;
@@ -415,25 +418,32 @@ __device__ __forceinline__ int lds_load(int* p)
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Mode-1 sweep of one tile: BX*BY compute threads (one column each) plus
-// one halo wave.  The halo wave polls the mailbox for the upwind columns'
-// values of the coming steps and writes them into the LDS ring; compute
-// waves wait on two LDS counters (halo steps ready, compute wave-steps
-// done) instead of workgroup barriers, and boundary columns post their
-// values to the mailbox as they compute them.  No progress flags and no
-// wait for store completion are needed: every mailbox word carries the
-// launch serial.
-template <typename R, bool INIT, int BX, int BY, int HB>
+// Mode-1 sweep of one tile: BX*BY compute threads (one column each), one
+// halo wave and one store wave.
+//   * The halo wave polls the mailbox for the upwind columns' values of the
+//     coming steps and writes them into the LDS ring.
+//   * Compute waves wait on LDS counters (halo steps ready, compute
+//     wave-steps done, store steps done) instead of workgroup barriers; they
+//     issue no global stores, so waiting for their prefetched loads never
+//     waits for stores (one counter covers both on this GPU).
+//   * The store wave copies each step's values from the ring to the field
+//     and posts tile-boundary values to the mailbox.
+// No progress flags and no wait for store completion are needed: every
+// mailbox word carries the launch serial.
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                                const int wg)
 {
+    // MODE 1: compute waves store their own values (two waves of support
+    // fewer per workgroup); MODE 2: a store wave does (see above)
+    constexpr bool SW = (MODE == 2);
     constexpr int NC = BX*BY;
     constexpr int NW = NC/64;
     constexpr int RING = 16;
     constexpr int HCOLS = BX + BY + 1;
     static_assert(NC % 64 == 0 && HCOLS <= 64 && HB + 3 <= RING, "tile");
     __shared__ R ring[RING][(BX + 1)*(BY + 1)];
-    __shared__ int s_cdone, s_hready, s_abort;
+    __shared__ int s_cdone, s_hready, s_sdone, s_abort;
     const int fbx = wg / a.nby, fby = wg - (wg / a.nby)*a.nby;
     const int fj0 = fbx*BX, fk0 = fby*BY;
     const int nz = a.nz, nx = a.nx, ny = a.ny;
@@ -445,8 +455,43 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     const int tlo = ez0 + jlo + klo;
     const int thi = a.fz1 + jhi + khi;
     const unsigned tag = (unsigned) a.serial;
-    if (threadIdx.x == 0) { s_cdone = 0; s_hready = tlo - 3; s_abort = 0; }
+    if (threadIdx.x == 0) {
+        s_cdone = 0; s_hready = tlo - 3; s_sdone = tlo - 1; s_abort = 0;
+    }
     __syncthreads();
+
+    if (SW && threadIdx.x >= NC + 64) {
+        // --------------------------------------------------- store wave
+        const int sl = threadIdx.x - NC - 64;
+        for (int t = tlo; t <= thi; t++) {
+            unsigned long long spins = 0;
+            while (lds_load(&s_cdone) < NW*(t - tlo + 1)) {
+                if (lds_load(&s_abort) || ++spins > kSpinLimit) return;
+            }
+            for (int cix = sl; cix < NC; cix += 64) {
+                const int lj = cix % BX, lk = cix / BX;
+                const int fj = fj0 + lj, fk = fk0 + lk;
+                const bool colE = (fj >= jlo && fj <= jhi && fk >= klo &&
+                                   fk <= khi);
+                const int fi = t - fj - fk;
+                if (!colE || fi < ez0 || fi > a.fz1) continue;
+                const bool colB = fj >= a.fx0 && fk >= a.fy0;
+                const bool post = (lj == BX - 1 || lk == BY - 1);
+                if (!(post || (colB && fi >= a.fz0))) continue;
+                const int j = unflip(fj, nx, a.sx), k = unflip(fk, ny, a.sy);
+                const long g = (long) (k - 1)*nz*nx + (long) (j - 1)*nz
+                             + ((a.sz > 0) ? fi : nz - 1 - fi);
+                const R v = ring[t & (RING - 1)][(lk + 1)*(BX + 1) + lj + 1];
+                if (colB && fi >= a.fz0) a.tt[g] = v;
+                if (post) mb_put(a.mb, g, v, tag);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            if (sl == 0)
+                __hip_atomic_store(&s_sdone, t, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        return;
+    }
 
     if (threadIdx.x >= NC) {
         // ---------------------------------------------------- halo wave
@@ -503,7 +548,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     const int cj = min(max(fj, jlo), jhi), ck = min(max(fk, klo), khi);
     const int j = unflip(cj, nx, a.sx), k = unflip(ck, ny, a.sy);
     const long colbase = (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
-    R* const col = a.tt + colbase;
+    const R* const col = a.tt + colbase;
     const int jm = max(j - 1, 1), jp = min(j, nx - 1);
     const int km = max(k - 1, 1), kp = min(k, ny - 1);
     const int xu = (a.sx > 0) ? jm : jp, xd = (a.sx > 0) ? jp : jm;
@@ -554,12 +599,16 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         for (int q = 0; q < HB; q++) {
             const int t = tb + q;
             if (t > thi) break;                       // workgroup-uniform
-            // halo of steps < t and every compute wave past step t-1
+            // halo of steps < t, every compute wave past step t-1, and the
+            // store wave done with the ring slot this step overwrites
             const int needC = NW*(t - tlo);
             unsigned long long spins = 0;
-            while (lds_load(&s_hready) < t || lds_load(&s_cdone) < needC) {
+            while (lds_load(&s_hready) < t || lds_load(&s_cdone) < needC ||
+                   (SW && lds_load(&s_sdone) < t - RING + 3)) {
                 if (lds_load(&s_abort) || ++spins > kSpinLimit) {
                     atomicExch(a.err, 1);
+                    __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                     return;
                 }
             }
@@ -589,9 +638,9 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                     }
                     dmax = rmax(dmax, v - vn);
                     v = vn;
-                    col[node_off(fi)] = v;
+                    if (!SW) a.tt[colbase + node_off(fi)] = v;
                 }
-                if (post) mb_put(a.mb, colbase + node_off(fi), v, tag);
+                if (!SW && post) mb_put(a.mb, colbase + node_off(fi), v, tag);
                 ring[t & (RING - 1)][me] = v;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
