@@ -49,6 +49,7 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
     a.serial = serial;
     a.maxdelta = track ? g.maxdelta : nullptr;
     a.mb = nullptr;
+    a.colmod = nullptr; a.plan = nullptr;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) {
         if (threadIdx.x == 0)
             __hip_atomic_store(flags + m.flag, (serial << 32) | 0xffffffffull,

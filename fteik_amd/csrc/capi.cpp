@@ -254,6 +254,9 @@ struct Solver3D {
     void *d_cst = nullptr, *d_cst32 = nullptr;
     unsigned long long *d_flags = nullptr;
     unsigned long long *d_mb = nullptr;  // tagged mailbox, 2 words per node
+    unsigned long long *d_colmod = nullptr, *d_lastproc = nullptr;
+    int *d_plan = nullptr;                     // tile skipping state
+    bool skipping = true;
     int *d_err = nullptr;
     double *d_maxdelta = nullptr;
     int *d_ri = nullptr;
@@ -274,6 +277,7 @@ void free3d()
     dfree(g3.d_cst); dfree(g3.d_cst32);
     dfree(g3.d_flags); dfree(g3.d_err); dfree(g3.d_maxdelta);
     dfree(g3.d_mb);
+    dfree(g3.d_colmod); dfree(g3.d_lastproc); dfree(g3.d_plan);
     dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr);
     if (g3.ev0) hipEventDestroy(g3.ev0);
     if (g3.ev1) hipEventDestroy(g3.ev1);
@@ -309,7 +313,19 @@ SweepLaunch launch3d(bool fp32)
     L.cooperative = g3.cooperative;
     L.tile = fp32 ? g3.tile32 : g3.tile;
     L.mb = g3.d_mb;
+    L.colmod = nullptr; L.lastproc = nullptr; L.plan = nullptr;
     return L;
+}
+
+// Every tile dirty: columns changed at serial 1, never processed.
+bool reset_skipping(SweepLaunch &L)
+{
+    if (!g3.skipping) return true;
+    L.colmod = g3.d_colmod; L.lastproc = g3.d_lastproc; L.plan = g3.d_plan;
+    return fteik_k_fill_u64(g3.d_colmod, (long) g3.nx*g3.ny, 1ull,
+                            g_dev.stream) == 0 &&
+           fteik_k_fill_u64(g3.d_lastproc, 8L*g3.max_flags, 0ull,
+                            g_dev.stream) == 0;
 }
 
 bool run_sweep(SweepLaunch &L, int sweep, bool linit)
@@ -542,6 +558,9 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
               dalloc((char **) &g3.d_cst32, 512) &&
               dalloc(&g3.d_flags, g3.max_flags) && dalloc(&g3.d_err, 1) &&
               dalloc(&g3.d_mb, 2*(size_t) ngrd) &&
+              dalloc(&g3.d_colmod, (size_t) nx*ny) &&
+              dalloc(&g3.d_lastproc, 8*(size_t) g3.max_flags) &&
+              dalloc(&g3.d_plan, 1 + 2*(size_t) g3.max_flags) &&
               dalloc(&g3.d_maxdelta, 1);
     if (!ok) {
         msg("fteik_solver3d_initialize: out of GPU memory");
@@ -563,11 +582,15 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     for (int t = 0; t < ntile; t++)
         g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
                                         fteik_k_pencil_fits(nx, ny, 1, t));
-    // default: the first available shape in order of measured speed (8x8
-    // mailbox with store wave, 8x8 mailbox, 16x8 ..., then flag tiles)
-    static const int pref[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
+    // default: the first available shape in order of measured speed.  When
+    // 16x8 tiles give every CU at least one workgroup (257^3: 512 tiles),
+    // 16x8 mailbox tiles are fastest; on smaller grids the 8x8 tiles keep
+    // more CUs busy.
+    static const int pref_large[] = {8, 11, 9, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
+    static const int pref_small[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
+    const bool large = (long) (nx - 1)*(ny - 1) >= 128L*g_dev.ncu;
     int first = -1;
-    for (int t : pref)
+    for (int t : (large ? pref_large : pref_small))
         if (t < ntile && g3.tileOK[(size_t) t]) { first = t; break; }
     g3.cooperative = first >= 0;
     g3.tile = g3.tile32 = first >= 0 ? first : 0;
@@ -797,6 +820,12 @@ void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr)
     *ntiles = (int) g3.tileOK.size();
 }
 
+void fteik_solver3d_setTileSkipping(int on, int *ierr)
+{
+    *ierr = 0;
+    g3.skipping = on != 0;
+}
+
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corr, double *updates,
                                      int *ierr)
 {
@@ -861,6 +890,7 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
     }
     int corr = 0;
     if (g3.precision == 0) {
+        ok = ok && reset_skipping(L);
         for (int k = 0; k < g3.nsweep && ok; k++)
             for (int sw = 1; sw <= 8 && ok; sw++) {
                 ok = run_sweep(L, sw, false);
@@ -892,6 +922,7 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
         if (ok) ok = fteik_k_min_merge_f(g3.d_tt, g3.d_tt32, ngrd,
                                          g3.mixedBias, g_dev.stream) == 0;
         L.maxdelta = g3.d_maxdelta;
+        ok = ok && reset_skipping(L);
         const int maxCorr = g3.maxCorr;
         for (corr = 0; corr < maxCorr && ok; ) {
             hipMemsetAsync(g3.d_maxdelta, 0, sizeof(double), g_dev.stream);

@@ -29,6 +29,12 @@ struct SweepLaunch {
     int cooperative;
     int tile;                    // tile shape index of the pipelined sweep
     unsigned long long* mb;      // mailbox of the mode-1 tiles
+    // tile skipping state (null: off): per-column serial of the last change
+    // (nx*ny), per-direction serial of each tile's last pass (8*max_flags),
+    // and the launch plan (1 + 2*max_flags ints)
+    unsigned long long* colmod;
+    unsigned long long* lastproc;
+    int* plan;
 };
 
 // One grid of a batched solve (batch3d.hip); device pointers.
@@ -78,6 +84,8 @@ int fteik_k_receivers(double* ttr, const double* tt, const int* ri,
                       const double* rd, int nrec, int nz, int nx, double dz,
                       double dx, double dy, int is3d, hipStream_t stream);
 int fteik_k_d2f(float* out, const double* in, long n, hipStream_t stream);
+int fteik_k_fill_u64(unsigned long long* a, long n, unsigned long long v,
+                     hipStream_t stream);
 int fteik_k_f2d_up(double* out, const float* in, long n, hipStream_t stream);
 int fteik_k_analytic(double* tt_xfast, int nz, int nx, int ny, double dz,
                      double dx, double dy, double z0, double x0, double y0,

@@ -93,13 +93,95 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
 }
 
 // compute waves + halo wave (+ store wave in mode 2); at least 3 waves per
-// SIMD so two workgroups share a CU
-template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
+// SIMD so two workgroups share a CU.  SKIP: iteration sweep with tile
+// skipping (a separate instance, so the dense sweep keeps its registers)
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE, bool SKIP>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(3)))
 void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
 {
-    pencil_mb_body<R, INIT, BX, BY, HB, MODE>(a, blockIdx.x);
+    if (SKIP) {
+        // the plan lists the processed tiles; the other workgroups exit
+        if ((int) blockIdx.x >= a.plan[0]) return;
+        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(
+            a, a.plan[1 + blockIdx.x]);
+    } else {
+        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(a, blockIdx.x);
+    }
+}
+
+// Tile-skipping plan of one iteration sweep (one workgroup).  A tile whose
+// columns have not changed since its last pass in this sweep direction
+// (colmod <= lastproc) and whose upwind tiles are all skipped would
+// reproduce its current values exactly: every node's update reads the same
+// upwind values as in that pass, and a sweep is idempotent on its own
+// output.  Unrolling "and whose upwind tiles are skipped" gives: a tile is
+// processed iff some tile of its upwind quadrant (fbx' <= fbx and
+// fby' <= fby) has a changed column -- a 2D prefix-OR over the tiles.
+// Writes the plan (count, processed tiles in tile order, skip flags) and
+// stamps every tile's lastproc with this launch's serial.
+constexpr int kPlanMaxTiles = 4096;
+
+// Tiles cover the box [fx0, fx1] x [fy0, fy1] (flipped columns) from its
+// first column; the column row before the box is read by the first tiles
+// and counts as theirs.
+__global__ __launch_bounds__(1024)
+void skip_plan_kernel(const unsigned long long* __restrict__ colmod,
+                      unsigned long long* __restrict__ lastproc,
+                      int* __restrict__ plan, int nx, int ny, int sx, int sy,
+                      int fx0, int fx1, int fy0, int fy1, int bx, int by,
+                      int nbx, int nby, unsigned long long serial)
+{
+    __shared__ unsigned long long s_last[kPlanMaxTiles];
+    __shared__ unsigned char s_dirty[kPlanMaxTiles];
+    const int nt = nbx*nby;
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+        s_last[t] = lastproc[t];
+        s_dirty[t] = 0;
+    }
+    __syncthreads();
+    const int lx = fx1 - fx0 + 2, ly = fy1 - fy0 + 2;
+    for (int c = threadIdx.x; c < lx*ly; c += blockDim.x) {
+        const int fk = fy0 - 1 + c / lx, fj = fx0 - 1 + c % lx;
+        const int t = (max(fj - fx0, 0)/bx)*nby + max(fk - fy0, 0)/by;
+        const int j = unflip(fj, nx, sx), k = unflip(fk, ny, sy);
+        if (colmod[(long) (k - 1)*nx + (j - 1)] > s_last[t]) s_dirty[t] = 1;
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < nbx; p += blockDim.x) {
+        unsigned char acc = 0;
+        for (int q = 0; q < nby; q++) {
+            acc |= s_dirty[p*nby + q];
+            s_dirty[p*nby + q] = acc;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nby; q += blockDim.x) {
+        unsigned char acc = 0;
+        for (int p = 0; p < nbx; p++) {
+            acc |= s_dirty[p*nby + q];
+            s_dirty[p*nby + q] = acc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // stable compaction of the processed tiles, 64 at a time
+        const int lane = threadIdx.x;
+        int base = 0;
+        for (int t0 = 0; t0 < nt; t0 += 64) {
+            const int t = t0 + lane;
+            const bool run = t < nt && s_dirty[t];
+            const unsigned long long m = __ballot(run);
+            if (run)
+                plan[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+            base += __popcll(m);
+        }
+        if (lane == 0) plan[0] = base;
+    }
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+        plan[1 + nt + t] = s_dirty[t] ? 0 : 1;
+        lastproc[t] = serial;
+    }
 }
 
 // Simple reference schedule: one launch per level (used for cross-checks
@@ -143,13 +225,14 @@ void sweep3d_level_kernel(SweepArgs<R> a, int L)
 }
 
 // ------------------------------------------------------------ launchers
-template <typename R, bool INIT>
+template <typename R, bool INIT, bool SKIP>
 static const void* pencil_fn_t(int tile)
 {
-#define PF(n) case n: return (const void*) sweep3d_pencil_kernel<R, INIT, \
+#define PF(n) case n: return SKIP ? nullptr \
+        : (const void*) sweep3d_pencil_kernel<R, INIT, \
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
 #define PM(n) case n: return (const void*) sweep3d_pencil_mb_kernel<R, INIT, \
-        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode, SKIP>;
     switch (tile) {
         PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
         PM(7) PM(8) PM(9) PM(10) PM(11) PM(12)
@@ -166,10 +249,69 @@ static int tile_threads(int tile)
     return kTiles[tile].bx*kTiles[tile].by + 64*kTiles[tile].mode;
 }
 
+// variant 0: iteration sweep, 1: init sweep, 2: iteration sweep with tile
+// skipping (mailbox tiles only)
 template <typename R>
-static const void* pencil_fn(bool linit, int tile)
+static const void* pencil_fn(int variant, int tile)
 {
-    return linit ? pencil_fn_t<R, true>(tile) : pencil_fn_t<R, false>(tile);
+    switch (variant) {
+        case 0: return pencil_fn_t<R, false, false>(tile);
+        case 1: return pencil_fn_t<R, true, false>(tile);
+        case 2: return pencil_fn_t<R, false, true>(tile);
+        default: return nullptr;
+    }
+}
+
+// Dynamic LDS that caps the workgroups per CU at ceil(nblk / CUs), so a
+// cooperative grid spreads evenly (one compute wave per SIMD where the tile
+// count allows) instead of stacking workgroups on some CUs while others
+// idle: the pipeline runs at the pace of its slowest tile.  Cached per
+// (kernel, workgroup count).
+static size_t spread_lds(const void* fn, int threads, int nblk)
+{
+#ifdef FTEIK_NO_SPREAD
+    (void) fn; (void) threads; (void) nblk;
+    return 0;
+#else
+    struct Entry { const void* fn; int nblk; size_t pad; };
+    static Entry cache[64];
+    static int ncache = 0;
+    for (int e = 0; e < ncache; e++)
+        if (cache[e].fn == fn && cache[e].nblk == nblk) return cache[e].pad;
+    int dev = 0, ncu = 0, lds_cu = 0, lds_blk = 0;
+    size_t pad = 0;
+    hipFuncAttributes fa;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds_cu,
+                              hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds_blk,
+                              hipDeviceAttributeMaxSharedMemoryPerBlock,
+                              dev) == hipSuccess &&
+        hipFuncGetAttributes(&fa, fn) == hipSuccess && ncu > 0) {
+        const int k = (nblk + ncu - 1)/ncu;
+        int natural = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&natural, fn, threads,
+                                                         0) == hipSuccess &&
+            natural > k) {
+            const size_t want = (size_t) lds_cu/(k + 1) + 1;
+            int got = 0;
+            if (want <= (size_t) lds_blk && want > fa.sharedSizeBytes &&
+                hipFuncSetAttribute(fn,
+                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int) (want - fa.sharedSizeBytes)) == hipSuccess &&
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &got, fn, threads, want - fa.sharedSizeBytes)
+                    == hipSuccess && got == k)
+                pad = want - fa.sharedSizeBytes;
+        }
+    }
+    (void) hipGetLastError();
+    if (ncache < 64) cache[ncache++] = {fn, nblk, pad};
+    return pad;
+#endif
 }
 
 template <typename R>
@@ -188,6 +330,7 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     a.fy0 = b[4]; a.fy1 = b[5];
     a.maxdelta = L.maxdelta;
     a.mb = L.mb;
+    a.colmod = nullptr; a.plan = nullptr;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) return 0;
     if (L.schedule == 1) {
         const int nlev = (a.fz1 + a.fx1 + a.fy1) - (a.fz0 + a.fx0 + a.fy0) + 1;
@@ -206,17 +349,36 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     }
     const int tile = (L.tile >= 0 && L.tile < kNumTiles) ? L.tile : 0;
     const TileShape ts = kTiles[tile];
-    a.nbx = (L.nx + ts.bx - 1)/ts.bx;
-    a.nby = (L.ny + ts.by - 1)/ts.by;
+    if (ts.mode == 0) {
+        // flag-exchange tiles cover the columns from 0
+        a.nbx = (L.nx + ts.bx - 1)/ts.bx;
+        a.nby = (L.ny + ts.by - 1)/ts.by;
+    } else {
+        // mailbox tiles cover the box's columns
+        a.nbx = (a.fx1 - a.fx0 + ts.bx)/ts.bx;
+        a.nby = (a.fy1 - a.fy0 + ts.by)/ts.by;
+    }
     const int nblk = a.nbx*a.nby;
     if (nblk > L.max_flags) return 2;
     void* args[] = {&a};
     hipError_t e;
-    const void* fn = pencil_fn<R>(linit, tile);
     if (ts.mode >= 1 && !L.mb) return 2;
+    const bool skip = ts.mode >= 1 && !linit && L.colmod && L.plan &&
+                      nblk <= kPlanMaxTiles;
+    if (skip) {
+        a.colmod = L.colmod; a.plan = L.plan;
+        hipLaunchKernelGGL(skip_plan_kernel, dim3(1), dim3(1024), 0, stream,
+                           L.colmod, L.lastproc + (long) (sweep - 1)*nblk,
+                           L.plan, L.nx, L.ny, a.sx, a.sy, a.fx0, a.fx1,
+                           a.fy0, a.fy1, ts.bx, ts.by, a.nbx, a.nby,
+                           L.serial);
+    }
+    const void* fn = pencil_fn<R>(linit ? 1 : (skip ? 2 : 0), tile);
     if (L.cooperative)
         e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(tile_threads(tile)),
-                                       args, 0, stream);
+                                       args, ts.mode >= 1 ?
+                                       spread_lds(fn, tile_threads(tile), nblk)
+                                       : 0, stream);
     else
         e = hipLaunchKernel(fn, dim3(nblk), dim3(tile_threads(tile)), args, 0,
                             stream);
@@ -244,9 +406,9 @@ extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                               dev) != hipSuccess) return 0;
-    for (int v = 0; v < 2; v++) {
-        const void* fn = fp32 ? pencil_fn<float>(v == 1, tile)
-                              : pencil_fn<double>(v == 1, tile);
+    for (int v = 0; v < (ts.mode >= 1 ? 3 : 2); v++) {
+        const void* fn = fp32 ? pencil_fn<float>(v, tile)
+                              : pencil_fn<double>(v, tile);
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
                                                          tile_threads(tile), 0)

@@ -87,6 +87,13 @@ struct SweepArgs {
     int nbx, nby;                // blocks along flipped x, y
     double* maxdelta;            // optional: max |t_new - t_old| (as bits)
     unsigned long long* mb;      // mailbox (mode 1): 2 words/node FP64, 1 FP32
+    // Tile skipping (mailbox tiles, iteration sweeps; null = off):
+    //   colmod[k*nx + j]  launch serial of the last change of column (j, k)
+    //   plan[0]           number of tiles processed in this launch
+    //   plan[1 + w]       tile of workgroup w (w < plan[0])
+    //   plan[1 + ntiles + t]  1 if tile t is skipped in this launch
+    unsigned long long* colmod;
+    const int* plan;
 };
 
 template <typename R>
@@ -430,10 +437,15 @@ __device__ __forceinline__ int lds_load(int* p)
 //     and posts tile-boundary values to the mailbox.
 // No progress flags and no wait for store completion are needed: every
 // mailbox word carries the launch serial.
-template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
+//
+// SKIP (iteration sweeps): only the tiles of a.plan run; a processed tile
+// stamps the columns it changed in a.colmod.
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE,
+          bool SKIP>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                                const int wg)
 {
+    static_assert(!(INIT && SKIP), "tile skipping is for iteration sweeps");
     // MODE 1: compute waves store their own values (two waves of support
     // fewer per workgroup); MODE 2: a store wave does (see above)
     constexpr bool SW = (MODE == 2);
@@ -444,13 +456,16 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     static_assert(NC % 64 == 0 && HCOLS <= 64 && HB + 3 <= RING, "tile");
     __shared__ R ring[RING][(BX + 1)*(BY + 1)];
     __shared__ int s_cdone, s_hready, s_sdone, s_abort;
+    // Tiles cover the box's columns from (fx0, fy0): the column row before
+    // the box (fx0 - 1 or fy0 - 1) is not updated in this launch and is
+    // read from the field.  At n = 257 the 256 updated columns per axis
+    // make exactly 256/BX by 256/BY tiles.
     const int fbx = wg / a.nby, fby = wg - (wg / a.nby)*a.nby;
-    const int fj0 = fbx*BX, fk0 = fby*BY;
+    const int fj0 = a.fx0 + fbx*BX, fk0 = a.fy0 + fby*BY;
     const int nz = a.nz, nx = a.nx, ny = a.ny;
     const int ez0 = a.fz0 - 1, ex0 = a.fx0 - 1, ey0 = a.fy0 - 1;
-    const int jlo = max(fj0, ex0), jhi = min(min(fj0 + BX - 1, a.fx1), nx - 1);
-    const int klo = max(fk0, ey0), khi = min(min(fk0 + BY - 1, a.fy1), ny - 1);
-    // a tile outside the extended box owns no node anyone reads
+    const int jlo = fj0, jhi = min(fj0 + BX - 1, a.fx1);
+    const int klo = fk0, khi = min(fk0 + BY - 1, a.fy1);
     if (jlo > jhi || klo > khi) return;
     const int tlo = ez0 + jlo + klo;
     const int thi = a.fz1 + jhi + khi;
@@ -502,39 +517,75 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         else if (hl < BY + BX)  { hj = fj0 + (hl - BY); hk = fk0 - 1;
                                   slot = hl - BY + 1; }
         else                    { hj = fj0 - 1; hk = fk0 - 1; slot = 0; }
-        // the producer writes exactly its extended-box columns
+        // halo columns inside the extended box (one column row before it)
         const bool hcol = hl < HCOLS && hj >= ex0 && hk >= ey0 &&
-                          hj <= min(a.fx1, nx - 1) && hk <= min(a.fy1, ny - 1);
+                          hj <= a.fx1 && hk <= a.fy1;
         const long gcol = hcol ? (long) (unflip(hk, ny, a.sy) - 1)*nz*nx
                                + (long) (unflip(hj, nx, a.sx) - 1)*nz : 0;
-        for (int s = tlo - 3; s <= thi - 1; s++) {
-            // the ring slot of step s was last read at step s - RING + 3
-            const int need = NW*(s - RING + 3 - tlo + 1);
-            unsigned long long spins = 0;
-            while (lds_load(&s_cdone) < need) {
-                if (++spins > kSpinLimit || lds_load(&s_abort)) return;
+        // columns outside the box, and those of a skipped upwind tile, are
+        // not written in this launch: they are read from the field instead
+        // of the mailbox
+        bool direct = hj < a.fx0 || hk < a.fy0;
+        if (SKIP && hcol && !direct) {
+            const int ut = ((hj - a.fx0)/BX)*a.nby + (hk - a.fy0)/BY;
+            direct = a.plan[1 + a.nbx*a.nby + ut] != 0;
+        }
+        // Values read from the field do not change during the launch: they
+        // are loaded HD steps ahead (a double-buffered register window) so
+        // their latency is not paid once per step.  Mailbox words are
+        // polled one step at a time.
+        constexpr int HD = 8;
+        const int s_beg = tlo - 3, s_end = thi - 1;
+        R cur[HD], nxt[HD];
+        auto fetch = [&](int s0, R* buf) {
+            #pragma unroll
+            for (int q = 0; q < HD; q++) {
+                const int fi = s0 + q - hj - hk;
+                buf[q] = R(0);
+                if (direct && hcol && fi >= ez0 && fi <= a.fz1)
+                    buf[q] = a.tt[gcol + ((a.sz > 0) ? fi : nz - 1 - fi)];
             }
-            const int fi = s - hj - hk;
-            if (hcol && fi >= ez0 && fi <= a.fz1) {
-                const long g = gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
-                R v;
-                spins = 0;
-                while (!mb_get(a.mb, g, tag, v)) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > kSpinLimit) {
-                        atomicExch(a.err, 1);
-                        __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                        break;
-                    }
+        };
+        fetch(s_beg, cur);
+        for (int s0 = s_beg; s0 <= s_end; s0 += HD) {
+            if (s0 + HD <= s_end) fetch(s0 + HD, nxt);
+            #pragma unroll
+            for (int q = 0; q < HD; q++) {
+                const int s = s0 + q;
+                if (s > s_end) break;                 // wave-uniform
+                // the ring slot of step s was last read at step s - RING + 3
+                const int need = NW*(s - RING + 3 - tlo + 1);
+                unsigned long long spins = 0;
+                while (lds_load(&s_cdone) < need) {
+                    if (++spins > kSpinLimit || lds_load(&s_abort)) return;
                 }
-                ring[s & (RING - 1)][slot] = v;
+                const int fi = s - hj - hk;
+                if (hcol && fi >= ez0 && fi <= a.fz1) {
+                    R v = cur[q];
+                    if (!direct) {
+                        const long g = gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
+                        spins = 0;
+                        while (!mb_get(a.mb, g, tag, v)) {
+                            __builtin_amdgcn_s_sleep(1);
+                            if (++spins > kSpinLimit) {
+                                atomicExch(a.err, 1);
+                                __hip_atomic_store(&s_abort, 1,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                                break;
+                            }
+                        }
+                    }
+                    ring[s & (RING - 1)][slot] = v;
+                }
+                if (lds_load(&s_abort)) return;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                if (hl == 0)
+                    __hip_atomic_store(&s_hready, s + 1, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            if (lds_load(&s_abort)) return;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-            if (hl == 0)
-                __hip_atomic_store(&s_hready, s + 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            #pragma unroll
+            for (int q = 0; q < HD; q++) cur[q] = nxt[q];
         }
         return;
     }
@@ -654,6 +705,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         atomicMax((unsigned long long*) a.maxdelta,
                   (unsigned long long) __double_as_longlong(dd));
     }
+    if (SKIP && colB && dmax > R(0))
+        a.colmod[(long) (k - 1)*nx + (j - 1)] = a.serial;
 }
 
 __host__ __device__ inline void sweep_signs(int sweep, int& sz, int& sx,

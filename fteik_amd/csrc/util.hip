@@ -368,6 +368,12 @@ __global__ void k_statics(float* __restrict__ ts, const float* __restrict__ tobs
     ts[i] = s;
 }
 
+__global__ void k_fill_u64(unsigned long long* __restrict__ a, long n,
+                           unsigned long long v)
+{
+    GRID_STRIDE(i, n) a[i] = v;
+}
+
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
 } // namespace
@@ -427,6 +433,13 @@ int fteik_k_receivers(double* ttr, const double* tt, const int* ri,
     if (nrec < 1) return 0;
     hipLaunchKernelGGL(k_receivers, dim3((nrec + TPB - 1)/TPB), dim3(TPB), 0,
                        s, ttr, tt, ri, rd, nrec, nz, nx, dz, dx, dy, is3d);
+    return ok();
+}
+
+int fteik_k_fill_u64(unsigned long long* a, long n, unsigned long long v,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblocks(n)), dim3(TPB), 0, s, a, n, v);
     return ok();
 }
 

@@ -106,6 +106,12 @@ void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
  * resident for this grid are accepted. */
 void fteik_solver3d_setTile(int tile, int tile32, int *ierr);
 void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
+/* EXTENSION: skip, in the Gauss-Seidel iterations, every tile of the
+ * pipelined sweep whose columns have not changed since its last pass in the
+ * same direction and whose upwind tiles are skipped too: such a pass would
+ * reproduce the tile's values exactly, so results are unchanged (on by
+ * default; mailbox tile shapes only). */
+void fteik_solver3d_setTileSkipping(int on, int *ierr);
 /* EXTENSION: device time of the last solve (ms, hipEvent), number of FP64
  * correction iterations run (mixed mode) and node updates performed. */
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corrIterations,

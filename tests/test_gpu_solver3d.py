@@ -276,3 +276,24 @@ def test_every_tile_shape_matches_oracle(precision):
         np.testing.assert_array_equal(g.field(), ref)
         tested += 1
     assert tested == nt.value          # every shape fits this grid
+
+
+def test_tile_skipping_is_exact():
+    """Many iterations (so later ones skip most tiles): the field equals the
+    oracle's, and equals the solver's own dense result, bit for bit."""
+    nz, nx, ny, h = 45, 52, 38, 15.0
+    vel = hetero(nz, nx, ny, h, 8)
+    zs, xs, ys = 11.2 * h, 40.4 * h, 20.7 * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, 8, 5.0)
+    L = lib()
+    ierr = c_int()
+    fields = []
+    for on in (1, 0):
+        g = GPU3D(nz, nx, ny, h, nsweep=8)
+        L.fteik_solver3d_setTileSkipping(on, byref(ierr))
+        g.set_cells(vel)
+        g.set_sources(zs, xs, ys)
+        assert g.solve(1) == 0
+        fields.append(g.field())
+    np.testing.assert_array_equal(fields[0], ref)
+    np.testing.assert_array_equal(fields[1], ref)
