@@ -84,6 +84,9 @@ def load_library(path=None):
         "fteik_solver3d_setMixedParameters": [d, i, _ip],
         "fteik_solver3d_setTile": [i, i, _ip],
         "fteik_solver3d_setTileSkipping": [i, _ip],
+        "fteik_solver3d_setSweepOverlap": [i, _ip],
+        "fteik_solver3d_setChainTile": [i, i, _ip],
+        "fteik_solver3d_getSweepOverlap": [_ip, _ip, _ip, _ip],
         "fteik_solver3d_getTile": [_ip, _ip, _ip, _ip],
         "fteik_solver3d_getLastSolveInfo": [_dp, _ip, _dp, _ip],
         "fteik_solver3d_free": [],

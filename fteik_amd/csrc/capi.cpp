@@ -257,6 +257,11 @@ struct Solver3D {
     unsigned long long *d_colmod = nullptr, *d_lastproc = nullptr;
     int *d_plan = nullptr;                     // tile skipping state
     bool skipping = true;
+    unsigned long long *d_prog = nullptr;      // sweep-chain progress words
+    int chainTile = -1, chainTile32 = -1;      // mode-2 tiles for chains
+    bool overlap = false;                      // chain iteration sweeps
+    bool overlapForced = false;                // ... on every grid size
+    int ncu_chain = 256;
     int *d_err = nullptr;
     double *d_maxdelta = nullptr;
     int *d_ri = nullptr;
@@ -278,6 +283,7 @@ void free3d()
     dfree(g3.d_flags); dfree(g3.d_err); dfree(g3.d_maxdelta);
     dfree(g3.d_mb);
     dfree(g3.d_colmod); dfree(g3.d_lastproc); dfree(g3.d_plan);
+    dfree(g3.d_prog);
     dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr);
     if (g3.ev0) hipEventDestroy(g3.ev0);
     if (g3.ev1) hipEventDestroy(g3.ev1);
@@ -314,6 +320,7 @@ SweepLaunch launch3d(bool fp32)
     L.tile = fp32 ? g3.tile32 : g3.tile;
     L.mb = g3.d_mb;
     L.colmod = nullptr; L.lastproc = nullptr; L.plan = nullptr;
+    L.prog = g3.d_prog;
     return L;
 }
 
@@ -326,6 +333,34 @@ bool reset_skipping(SweepLaunch &L)
                             g_dev.stream) == 0 &&
            fteik_k_fill_u64(g3.d_lastproc, 8L*g3.max_flags, 0ull,
                             g_dev.stream) == 0;
+}
+
+// nsweep iterations of the 8 sweep directions.  With sweep chains, every
+// iteration sweep of the range runs in one launch, each sweep overlapping
+// the one before it; otherwise one launch per sweep.
+bool run_chain(SweepLaunch L, int niter)
+{
+    const int tile = L.fp32 ? g3.chainTile32 : g3.chainTile;
+    L.tile = tile;
+    L.serial = g3.serial + 1;
+    g3.serial += 8*(unsigned long long) niter;
+    L.colmod = nullptr; L.lastproc = nullptr; L.plan = nullptr;
+    return fteik_k_sweep3d_chain(&L, 0, 8*niter, g_dev.stream) == 0;
+}
+
+// Chains pay where one sweep leaves CUs idle (a sweep's tiles are active
+// only while the wavefront crosses them): at most one tile per CU.  On
+// larger grids every SIMD already holds a sweep's compute wave.  A chain
+// cannot skip tiles, so once a solve converges early tile skipping is
+// faster: chains are opt-in.
+bool chain_ok(bool fp32)
+{
+    const int tile = fp32 ? g3.chainTile32 : g3.chainTile;
+    if (!g3.overlap || g3.schedule != 0 || tile < 0) return false;
+    int bx = 0, by = 0, hb = 0, mode = 0;
+    fteik_k_tile_shape(tile, &bx, &by, &hb, &mode);
+    const long ntile = (long) ((g3.nx - 1 + bx - 1)/bx)*((g3.ny - 1 + by - 1)/by);
+    return g3.overlapForced || ntile <= g3.ncu_chain;
 }
 
 bool run_sweep(SweepLaunch &L, int sweep, bool linit)
@@ -561,6 +596,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
               dalloc(&g3.d_colmod, (size_t) nx*ny) &&
               dalloc(&g3.d_lastproc, 8*(size_t) g3.max_flags) &&
               dalloc(&g3.d_plan, 1 + 2*(size_t) g3.max_flags) &&
+              dalloc(&g3.d_prog, 2*(size_t) g3.max_flags) &&
               dalloc(&g3.d_maxdelta, 1);
     if (!ok) {
         msg("fteik_solver3d_initialize: out of GPU memory");
@@ -573,6 +609,8 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     hipMemsetAsync(g3.d_err, 0, sizeof(int), g_dev.stream);
     hipMemsetAsync(g3.d_mb, 0, 2*(size_t) ngrd*sizeof(unsigned long long),
                    g_dev.stream);
+    hipMemsetAsync(g3.d_prog, 0, 2*(size_t) g3.max_flags*
+                   sizeof(unsigned long long), g_dev.stream);
     hipEventCreate(&g3.ev0);
     hipEventCreate(&g3.ev1);
     // tile shapes whose whole grid of workgroups can be co-resident (FP64
@@ -594,6 +632,17 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
         if (t < ntile && g3.tileOK[(size_t) t]) { first = t; break; }
     g3.cooperative = first >= 0;
     g3.tile = g3.tile32 = first >= 0 ? first : 0;
+    // sweep chains: the first mode-2 tile with two sweeps' tiles resident
+    g3.ncu_chain = g_dev.ncu > 0 ? g_dev.ncu : 256;
+    g3.overlapForced = false;
+    static const int chain_pref[] = {13, 11, 10, 12};
+    g3.chainTile = g3.chainTile32 = -1;
+    for (int t : chain_pref) {
+        if (g3.chainTile < 0 && fteik_k_chain_fits(nx, ny, 0, t))
+            g3.chainTile = t;
+        if (g3.chainTile32 < 0 && fteik_k_chain_fits(nx, ny, 1, t))
+            g3.chainTile32 = t;
+    }
     if (!g3.cooperative) g3.schedule = 1;
     if (!sync_ok()) { msg("fteik_solver3d_initialize: GPU error"); *ierr = 1;
                       return; }
@@ -826,6 +875,41 @@ void fteik_solver3d_setTileSkipping(int on, int *ierr)
     g3.skipping = on != 0;
 }
 
+void fteik_solver3d_setSweepOverlap(int on, int *ierr)
+{
+    *ierr = 0;
+    if (on < 0 || on > 2) {
+        msg("fteik_solver3d_setSweepOverlap: on must be 0, 1 or 2");
+        *ierr = 1;
+        return;
+    }
+    g3.overlap = on != 0;
+    g3.overlapForced = on == 2;
+}
+
+void fteik_solver3d_setChainTile(int tile, int tile32, int *ierr)
+{
+    *ierr = 0;
+    if (!g3.init || !fteik_k_chain_fits(g3.nx, g3.ny, 0, tile) ||
+        !fteik_k_chain_fits(g3.nx, g3.ny, 1, tile32)) {
+        msg("fteik_solver3d_setChainTile: tile shapes (%d, %d) cannot chain "
+            "sweeps on this grid", tile, tile32);
+        *ierr = 1;
+        return;
+    }
+    g3.chainTile = tile;
+    g3.chainTile32 = tile32;
+}
+
+void fteik_solver3d_getSweepOverlap(int *on, int *tile, int *tile32,
+                                    int *ierr)
+{
+    *ierr = g3.init ? 0 : 1;
+    *on = chain_ok(false) ? 1 : 0;
+    *tile = g3.chainTile;
+    *tile32 = g3.chainTile32;
+}
+
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corr, double *updates,
                                      int *ierr)
 {
@@ -889,7 +973,12 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
         updates += sweep_updates(sw, true, s);
     }
     int corr = 0;
-    if (g3.precision == 0) {
+    if (g3.precision == 0 && chain_ok(false)) {
+        if (ok && g3.nsweep > 0) ok = run_chain(L, g3.nsweep);
+        for (int k = 0; k < g3.nsweep; k++)
+            for (int sw = 1; sw <= 8; sw++)
+                updates += sweep_updates(sw, false, s);
+    } else if (g3.precision == 0) {
         ok = ok && reset_skipping(L);
         for (int k = 0; k < g3.nsweep && ok; k++)
             for (int sw = 1; sw <= 8 && ok; sw++) {
@@ -912,11 +1001,18 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
         }
         SweepLaunch L32 = launch3d(true);
         L32.zsi = s.zsi; L32.xsi = s.xsi; L32.ysi = s.ysi;
-        for (int k = 0; k < g3.nsweep && ok; k++)
-            for (int sw = 1; sw <= 8 && ok; sw++) {
-                ok = run_sweep(L32, sw, false);
-                updates += sweep_updates(sw, false, s);
-            }
+        if (chain_ok(true)) {
+            if (ok && g3.nsweep > 0) ok = run_chain(L32, g3.nsweep);
+            for (int k = 0; k < g3.nsweep; k++)
+                for (int sw = 1; sw <= 8; sw++)
+                    updates += sweep_updates(sw, false, s);
+        } else {
+            for (int k = 0; k < g3.nsweep && ok; k++)
+                for (int sw = 1; sw <= 8 && ok; sw++) {
+                    ok = run_sweep(L32, sw, false);
+                    updates += sweep_updates(sw, false, s);
+                }
+        }
         // merge: the FP64 initialization values are upper bounds of the
         // converged field, so keep the smaller of the two
         if (ok) ok = fteik_k_min_merge_f(g3.d_tt, g3.d_tt32, ngrd,
@@ -924,11 +1020,18 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
         L.maxdelta = g3.d_maxdelta;
         ok = ok && reset_skipping(L);
         const int maxCorr = g3.maxCorr;
+        const bool chain = chain_ok(false);
         for (corr = 0; corr < maxCorr && ok; ) {
             hipMemsetAsync(g3.d_maxdelta, 0, sizeof(double), g_dev.stream);
-            for (int sw = 1; sw <= 8 && ok; sw++) {
-                ok = run_sweep(L, sw, false);
-                updates += sweep_updates(sw, false, s);
+            if (chain) {
+                ok = run_chain(L, 1);
+                for (int sw = 1; sw <= 8; sw++)
+                    updates += sweep_updates(sw, false, s);
+            } else {
+                for (int sw = 1; sw <= 8 && ok; sw++) {
+                    ok = run_sweep(L, sw, false);
+                    updates += sweep_updates(sw, false, s);
+                }
             }
             corr++;
             double md = 0.0;

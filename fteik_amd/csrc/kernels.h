@@ -35,6 +35,9 @@ struct SweepLaunch {
     unsigned long long* colmod;
     unsigned long long* lastproc;
     int* plan;
+    // sweep chains: progress words of the two workgroup sets
+    // (2*max_flags, zeroed once; serials only grow)
+    unsigned long long* prog;
 };
 
 // One grid of a batched solve (batch3d.hip); device pointers.
@@ -59,6 +62,10 @@ int fteik_k_batch_sweep(const fteik::BatchGrid* grids, const int* map,
 int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
                     hipStream_t stream);
 int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile);
+int fteik_k_chain_fits(int nx, int ny, int fp32, int tile);
+void fteik_k_tile_shape(int tile, int* bx, int* by, int* hb, int* mode);
+int fteik_k_sweep3d_chain(const fteik::SweepLaunch* L, int dir0, int nq,
+                          hipStream_t stream);
 int fteik_k_num_tiles(void);
 int fteik_k_tile_mode(int tile);
 int fteik_k_max_flags(int nx, int ny);

@@ -5,6 +5,7 @@
 // (every tile resident) so waits between tiles always end; schedule 1 is a
 // one-launch-per-level cross-check and fallback.
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include <stdint.h>
 #include "sweep_pencil.h"
 
@@ -97,7 +98,7 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
 // skipping (a separate instance, so the dense sweep keeps its registers)
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE, bool SKIP>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
-__attribute__((amdgpu_waves_per_eu(3)))
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
 void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
 {
     if (SKIP) {
@@ -108,6 +109,36 @@ void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
     } else {
         pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(a, blockIdx.x);
     }
+}
+
+// A chain of nq consecutive iteration sweeps (directions dir0 + q, q = 0..
+// nq-1, cyclically; serials a.serial + q) in one launch: workgroup
+// q*ntile + tile sweeps that tile in sweep q, overlapping sweep q - 1 as
+// far as the data dependences between them allow (see the CHAIN body).
+// The iteration-sweep box is the same in every direction in flipped
+// coordinates, so the tiles are too.  The launch need not be resident at
+// once: a workgroup waits only on lower-numbered ones (its sweep's upwind
+// tiles, the preceding sweep), and workgroups are dispatched in order, so
+// the lowest unfinished workgroup is always resident and never blocked.
+// Consecutive sweeps alternate between two progress-word sets.
+template <typename R, int BX, int BY, int HB>
+__global__ __launch_bounds__(BX*BY + 128)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
+void sweep3d_chain_kernel(SweepArgs<R> a, int dir0)
+{
+    const int ntile = a.nbx*a.nby;
+    const int q = (int) blockIdx.x / ntile;
+    const int wg = (int) blockIdx.x - q*ntile;
+    SweepArgs<R> b = a;
+    sweep_signs((dir0 + q) % 8 + 1, b.sz, b.sx, b.sy);
+    b.serial = a.serial + q;
+    b.par = q & 1;
+    b.pserial = 0;
+    if (q > 0) {
+        sweep_signs((dir0 + q - 1) % 8 + 1, b.psz, b.psx, b.psy);
+        b.pserial = a.serial + q - 1;
+    }
+    pencil_mb_body<R, false, BX, BY, HB, 2, false, true>(b, wg);
 }
 
 // Tile-skipping plan of one iteration sweep (one workgroup).  A tile whose
@@ -235,13 +266,13 @@ static const void* pencil_fn_t(int tile)
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode, SKIP>;
     switch (tile) {
         PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
-        PM(7) PM(8) PM(9) PM(10) PM(11) PM(12)
+        PM(7) PM(8) PM(9) PM(10) PM(11) PM(12) PM(13)
         default: return nullptr;
     }
 #undef PF
 #undef PM
 }
-static_assert(kNumTiles == 13 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
+static_assert(kNumTiles == 14 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
               "pencil_fn_t lists every tile");
 
 static int tile_threads(int tile)
@@ -260,58 +291,6 @@ static const void* pencil_fn(int variant, int tile)
         case 2: return pencil_fn_t<R, false, true>(tile);
         default: return nullptr;
     }
-}
-
-// Dynamic LDS that caps the workgroups per CU at ceil(nblk / CUs), so a
-// cooperative grid spreads evenly (one compute wave per SIMD where the tile
-// count allows) instead of stacking workgroups on some CUs while others
-// idle: the pipeline runs at the pace of its slowest tile.  Cached per
-// (kernel, workgroup count).
-static size_t spread_lds(const void* fn, int threads, int nblk)
-{
-#ifdef FTEIK_NO_SPREAD
-    (void) fn; (void) threads; (void) nblk;
-    return 0;
-#else
-    struct Entry { const void* fn; int nblk; size_t pad; };
-    static Entry cache[64];
-    static int ncache = 0;
-    for (int e = 0; e < ncache; e++)
-        if (cache[e].fn == fn && cache[e].nblk == nblk) return cache[e].pad;
-    int dev = 0, ncu = 0, lds_cu = 0, lds_blk = 0;
-    size_t pad = 0;
-    hipFuncAttributes fa;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
-                              dev) == hipSuccess &&
-        hipDeviceGetAttribute(&lds_cu,
-                              hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                              dev) == hipSuccess &&
-        hipDeviceGetAttribute(&lds_blk,
-                              hipDeviceAttributeMaxSharedMemoryPerBlock,
-                              dev) == hipSuccess &&
-        hipFuncGetAttributes(&fa, fn) == hipSuccess && ncu > 0) {
-        const int k = (nblk + ncu - 1)/ncu;
-        int natural = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&natural, fn, threads,
-                                                         0) == hipSuccess &&
-            natural > k) {
-            const size_t want = (size_t) lds_cu/(k + 1) + 1;
-            int got = 0;
-            if (want <= (size_t) lds_blk && want > fa.sharedSizeBytes &&
-                hipFuncSetAttribute(fn,
-                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int) (want - fa.sharedSizeBytes)) == hipSuccess &&
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &got, fn, threads, want - fa.sharedSizeBytes)
-                    == hipSuccess && got == k)
-                pad = want - fa.sharedSizeBytes;
-        }
-    }
-    (void) hipGetLastError();
-    if (ncache < 64) cache[ncache++] = {fn, nblk, pad};
-    return pad;
-#endif
 }
 
 template <typename R>
@@ -376,9 +355,7 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     const void* fn = pencil_fn<R>(linit ? 1 : (skip ? 2 : 0), tile);
     if (L.cooperative)
         e = hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(tile_threads(tile)),
-                                       args, ts.mode >= 1 ?
-                                       spread_lds(fn, tile_threads(tile), nblk)
-                                       : 0, stream);
+                                       args, 0, stream);
     else
         e = hipLaunchKernel(fn, dim3(nblk), dim3(tile_threads(tile)), args, 0,
                             stream);
@@ -396,6 +373,102 @@ extern "C" int fteik_k_sweep3d(const SweepLaunch* L, int sweep, int linit,
     return launch_sweep_t<double>(*L, sweep, linit != 0, stream);
 }
 
+// Workgroups of kernel fn that the GPU keeps resident per CU.  The
+// occupancy query counts registers per CU, but a workgroup's waves are
+// placed on the SIMDs starting from the first one: a workgroup of more than
+// four waves puts two on the first SIMD, and a second one fits only if that
+// SIMD has room for four such waves (measured: 5- and 6-wave workgroups at
+// 152 registers run one per CU although the query reports two).
+static int resident_per_cu(const void* fn, int threads)
+{
+    int api = 0;
+    hipFuncAttributes fa;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&api, fn, threads, 0)
+            != hipSuccess ||
+        hipFuncGetAttributes(&fa, fn) != hipSuccess) return 0;
+    const int waves = (threads + 63)/64;
+    if (waves <= 4) return api;
+    const int regs = ((fa.numRegs + 7)/8)*8;
+    const int per_simd = regs > 0 ? min(8, 512/regs) : 8;
+    return min(api, per_simd/((waves + 3)/4));
+}
+
+template <typename R>
+static const void* chain_fn(int tile)
+{
+#define PC(n) case n: return (const void*) sweep3d_chain_kernel<R, \
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
+    switch (tile) {
+        PC(10) PC(11) PC(12) PC(13)
+        default: return nullptr;
+    }
+#undef PC
+}
+static_assert(kTiles[10].mode == 2 && kTiles[11].mode == 2 &&
+              kTiles[12].mode == 2 && kTiles[13].mode == 2 && kNumTiles == 14,
+              "chain_fn lists the mode-2 tiles");
+
+// Tiles of the iteration-sweep box (flipped columns 1 .. n - 1).
+static void chain_tiles(int nx, int ny, const TileShape& ts, int& nbx,
+                        int& nby)
+{
+    nbx = (nx - 1 + ts.bx - 1)/ts.bx;
+    nby = (ny - 1 + ts.by - 1)/ts.by;
+}
+
+extern "C" int fteik_k_chain_fits(int nx, int ny, int fp32, int tile)
+{
+    // a sweep's tiles must be resident together with at least one more
+    // sweep's for the chain to overlap anything
+    const void* fn = fp32 ? chain_fn<float>(tile) : chain_fn<double>(tile);
+    if (!fn || nx < 2 || ny < 2) return 0;
+    const TileShape ts = kTiles[tile];
+    int nbx, nby, ncu = 0, dev = 0;
+    chain_tiles(nx, ny, ts, nbx, nby);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) return 0;
+    const int per_cu = resident_per_cu(fn, tile_threads(tile));
+    return (long) per_cu*ncu >= 2L*nbx*nby ? 1 : 0;
+}
+
+template <typename R>
+static int launch_chain_t(const SweepLaunch& L, int dir0, int nq,
+                          hipStream_t stream)
+{
+    const void* fn = chain_fn<R>(L.tile);
+    if (!fn || !L.mb || !L.prog || nq < 1 || dir0 < 0 || dir0 > 7) return 2;
+    const TileShape ts = kTiles[L.tile];
+    SweepArgs<R> a;
+    memset(&a, 0, sizeof(a));
+    a.tt = (R*) L.tt; a.slow = (const R*) L.slow;
+    a.cst = (const Consts<R>*) L.cst;
+    a.flags = L.flags; a.err = L.err; a.serial = L.serial;
+    a.nz = L.nz; a.nx = L.nx; a.ny = L.ny;
+    int b[6];
+    flipped_box(dir0 + 1, false, L.nz, L.nx, L.ny, 1, 1, 1, b);
+    a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
+    a.fy0 = b[4]; a.fy1 = b[5];
+    if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) return 0;
+    chain_tiles(L.nx, L.ny, ts, a.nbx, a.nby);
+    if (a.nbx*a.nby > L.max_flags) return 2;
+    a.maxdelta = L.maxdelta;
+    a.mb = L.mb;
+    a.prog = L.prog;
+    void* args[] = {&a, &dir0};
+    hipError_t e = hipLaunchKernel(fn, dim3(nq*a.nbx*a.nby),
+                                   dim3(tile_threads(L.tile)), args, 0,
+                                   stream);
+    return e == hipSuccess ? 0 : 1;
+}
+
+extern "C" int fteik_k_sweep3d_chain(const SweepLaunch* L, int dir0, int nq,
+                                     hipStream_t stream)
+{
+    if (L->fp32) return launch_chain_t<float>(*L, dir0, nq, stream);
+    return launch_chain_t<double>(*L, dir0, nq, stream);
+}
+
 extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
 {
     // cooperative launch needs every block of the grid resident at once
@@ -409,16 +482,20 @@ extern "C" int fteik_k_pencil_fits(int nx, int ny, int fp32, int tile)
     for (int v = 0; v < (ts.mode >= 1 ? 3 : 2); v++) {
         const void* fn = fp32 ? pencil_fn<float>(v, tile)
                               : pencil_fn<double>(v, tile);
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn,
-                                                         tile_threads(tile), 0)
-            != hipSuccess) return 0;
+        const int per_cu = resident_per_cu(fn, tile_threads(tile));
         if ((long) per_cu*ncu < nblk) return 0;
     }
     return 1;
 }
 
 extern "C" int fteik_k_num_tiles(void) { return kNumTiles; }
+
+extern "C" void fteik_k_tile_shape(int tile, int* bx, int* by, int* hb,
+                                   int* mode)
+{
+    const TileShape ts = kTiles[(tile >= 0 && tile < kNumTiles) ? tile : 0];
+    *bx = ts.bx; *by = ts.by; *hb = ts.hb; *mode = ts.mode;
+}
 
 extern "C" int fteik_k_tile_mode(int tile)
 {

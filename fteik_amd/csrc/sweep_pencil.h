@@ -27,9 +27,10 @@
 
 namespace fteik {
 
-// Tile shapes of the pipelined sweep: BX x BY columns per workgroup, halo
-// exchanged with the upwind workgroups every HB steps.  The autotuner picks
-// one per grid (fteik_solver3d_autotune); kTiles[0] is the default.
+// Tile shapes of the pipelined sweep: BX x BY columns per workgroup; HB is
+// the halo batch (mode 0) or the prefetch depth of the compute waves (modes
+// 1, 2).  The library picks one per grid at initialize; the autotuner
+// (fteik_solver3d_autotune) times them.
 // mode 0: upwind tiles publish a progress flag every hb steps and the
 //         downwind tile pulls its halo after the flag (pencil_body);
 // mode 1: boundary values go to a tagged mailbox and a dedicated halo wave
@@ -42,7 +43,14 @@ constexpr TileShape kTiles[] = {
     {8, 8, 2, 0}, {32, 8, 4, 0}, {16, 8, 4, 0},
     {16, 16, 4, 1}, {16, 8, 4, 1}, {8, 8, 4, 1},
     {16, 16, 4, 2}, {16, 8, 4, 2}, {8, 8, 4, 2},
+    // 4 waves in at most 128 registers: four workgroups per CU, so two
+    // sweeps' tiles of a 257^2 chain are resident together
+    {16, 8, 2, 2},
 };
+// Waves per SIMD the kernels of a tile shape are compiled for.  The
+// hardware places a workgroup's waves on the SIMDs from the first one, so a
+// 5- or 6-wave workgroup at 3 waves per SIMD leaves no room for a second.
+constexpr int waves_per_simd(int hb) { return hb <= 2 ? 4 : 3; }
 constexpr int kNumTiles = (int) (sizeof(kTiles)/sizeof(kTiles[0]));
 constexpr unsigned long long kSpinLimit = 1ull << 22;
 
@@ -94,7 +102,45 @@ struct SweepArgs {
     //   plan[1 + ntiles + t]  1 if tile t is skipped in this launch
     unsigned long long* colmod;
     const int* plan;
+    // Sweep chains (consecutive iteration sweeps in one launch; sweep q
+    // uses progress set par = q & 1):
+    //   prog[par*ntiles + tile] = (serial << 32) | last step stored, or
+    //                     (serial << 32) | kDone once the tile is finished
+    //   pserial           serial of the preceding sweep (0: none)
+    //   psz, psx, psy     its signs
+    unsigned long long* prog;
+    unsigned long long pserial;
+    int par;
+    int psz, psx, psy;
 };
+
+constexpr unsigned kDone = 0x7fffffffu;
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for
+constexpr int kWaitVmcnt0 = 0x0f70;
+
+// Device-coherent field access for sweep chains: the preceding sweep of a
+// chain runs in the same launch on other CUs (other L2s).
+__device__ __forceinline__ double ld_coh(const double* p)
+{
+    return __longlong_as_double((long long) __hip_atomic_load(
+        (unsigned long long*) p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ float ld_coh(const float* p)
+{
+    return __uint_as_float(__hip_atomic_load(
+        (unsigned*) p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_coh(double* p, double v)
+{
+    __hip_atomic_store((unsigned long long*) p,
+                       (unsigned long long) __double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(float* p, float v)
+{
+    __hip_atomic_store((unsigned*) p, __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
 
 template <typename R>
 __device__ __forceinline__ void slowness7(const R* __restrict__ slow, int nz,
@@ -425,27 +471,46 @@ __device__ __forceinline__ int lds_load(int* p)
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Mode-1 sweep of one tile: BX*BY compute threads (one column each), one
-// halo wave and one store wave.
+// Mailbox sweep of one tile: BX*BY compute threads (one column each), one
+// halo wave and, in mode 2, one store wave.
 //   * The halo wave polls the mailbox for the upwind columns' values of the
 //     coming steps and writes them into the LDS ring.
-//   * Compute waves wait on LDS counters (halo steps ready, compute
-//     wave-steps done, store steps done) instead of workgroup barriers; they
-//     issue no global stores, so waiting for their prefetched loads never
-//     waits for stores (one counter covers both on this GPU).
-//   * The store wave copies each step's values from the ring to the field
-//     and posts tile-boundary values to the mailbox.
+//   * Compute waves wait on LDS step counters instead of workgroup
+//     barriers: wave w needs the halo and wave w - 1 (the rows before its
+//     own), so the waves of a tile pipeline.
+//   * Mode 1: compute waves store their values and post tile-boundary
+//     values themselves.  Mode 2: the store wave copies each step's values
+//     from the ring to the field and the mailbox, so compute waves issue no
+//     global stores and waiting for their prefetched loads never waits for
+//     stores (one counter covers both on this GPU).
 // No progress flags and no wait for store completion are needed: every
 // mailbox word carries the launch serial.
 //
 // SKIP (iteration sweeps): only the tiles of a.plan run; a processed tile
 // stamps the columns it changed in a.colmod.
+//
+// CHAIN (iteration sweeps, mode 2): the preceding sweep of the chain may
+// still be running.  Node m may be read or overwritten once the preceding
+// sweep A has finalized it and read it for the last time, i.e. once A has
+// passed level L_A(m) + 3 in the A tiles holding m's column and its
+// neighbours.  Over the nodes a tile touches at step t (its columns, the
+// halo row before them and the one after, node and z-upwind node) that
+// bound is
+//     z as in A:  t + 3 + sum over x, y flipped of (n + 1 - 2 f0)
+//     z flipped:  n_z - t_first + 3 + sum over x, y of (2 (f0 + B) if as in
+//                 A, else n - 1)
+// (f0 = the tile's first flipped column, B = its width).  The halo wave
+// polls the progress words of those A tiles (at most 3 x 3) ahead of the
+// steps it releases.  The store wave publishes the tile's own progress with
+// a release after its stores; field loads and stores are device-coherent.
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE,
-          bool SKIP>
+          bool SKIP, bool CHAIN = false>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                                const int wg)
 {
     static_assert(!(INIT && SKIP), "tile skipping is for iteration sweeps");
+    static_assert(!CHAIN || (!INIT && !SKIP && MODE == 2),
+                  "sweep chains: iteration sweeps of mode-2 tiles");
     // MODE 1: compute waves store their own values (two waves of support
     // fewer per workgroup); MODE 2: a store wave does (see above)
     constexpr bool SW = (MODE == 2);
@@ -455,7 +520,11 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     constexpr int HCOLS = BX + BY + 1;
     static_assert(NC % 64 == 0 && HCOLS <= 64 && HB + 3 <= RING, "tile");
     __shared__ R ring[RING][(BX + 1)*(BY + 1)];
-    __shared__ int s_cdone, s_hready, s_sdone, s_abort;
+    // s_wdone[w]: last step compute wave w completed (wave w reads only
+    // wave w-1's cells and the halo, so waves pipeline instead of stepping
+    // together); s_hready: halo released through step s_hready - 1;
+    // s_sdone: last step the store wave wrote out
+    __shared__ int s_wdone[NW], s_hready, s_sdone, s_abort;
     // Tiles cover the box's columns from (fx0, fy0): the column row before
     // the box (fx0 - 1 or fy0 - 1) is not updated in this launch and is
     // read from the field.  At n = 257 the 256 updated columns per axis
@@ -471,39 +540,75 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     const int thi = a.fz1 + jhi + khi;
     const unsigned tag = (unsigned) a.serial;
     if (threadIdx.x == 0) {
-        s_cdone = 0; s_hready = tlo - 3; s_sdone = tlo - 1; s_abort = 0;
+        for (int w = 0; w < NW; w++) s_wdone[w] = tlo - 1;
+        s_hready = tlo - 3; s_sdone = tlo - 1; s_abort = 0;
     }
     __syncthreads();
+    auto wdone_min = [&]() -> int {
+        int m = lds_load(&s_wdone[0]);
+        #pragma unroll
+        for (int w = 1; w < NW; w++) m = min(m, lds_load(&s_wdone[w]));
+        return m;
+    };
 
     if (SW && threadIdx.x >= NC + 64) {
         // --------------------------------------------------- store wave
+        // NW columns per lane; their field offsets, diagonal and ring cell
+        // are computed once
         const int sl = threadIdx.x - NC - 64;
+        long sbase[NW];
+        int sdep[NW], sring[NW], sflag[NW];
+        #pragma unroll
+        for (int c = 0; c < NW; c++) {
+            const int cix = sl + 64*c;
+            const int lj = cix % BX, lk = cix / BX;
+            const int fj = fj0 + lj, fk = fk0 + lk;
+            const bool colE = (fj >= jlo && fj <= jhi && fk >= klo &&
+                               fk <= khi);
+            const bool post = (lj == BX - 1 || lk == BY - 1);
+            sflag[c] = colE ? (1 | (post ? 2 : 0)) : 0;
+            sdep[c] = fj + fk;
+            sring[c] = (lk + 1)*(BX + 1) + lj + 1;
+            sbase[c] = colE ? (long) (unflip(fk, ny, a.sy) - 1)*nz*nx
+                              + (long) (unflip(fj, nx, a.sx) - 1)*nz : 0;
+        }
         for (int t = tlo; t <= thi; t++) {
             unsigned long long spins = 0;
-            while (lds_load(&s_cdone) < NW*(t - tlo + 1)) {
+            while (wdone_min() < t) {
                 if (lds_load(&s_abort) || ++spins > kSpinLimit) return;
+                __builtin_amdgcn_s_sleep(1);
             }
-            for (int cix = sl; cix < NC; cix += 64) {
-                const int lj = cix % BX, lk = cix / BX;
-                const int fj = fj0 + lj, fk = fk0 + lk;
-                const bool colE = (fj >= jlo && fj <= jhi && fk >= klo &&
-                                   fk <= khi);
-                const int fi = t - fj - fk;
-                if (!colE || fi < ez0 || fi > a.fz1) continue;
-                const bool colB = fj >= a.fx0 && fk >= a.fy0;
-                const bool post = (lj == BX - 1 || lk == BY - 1);
-                if (!(post || (colB && fi >= a.fz0))) continue;
-                const int j = unflip(fj, nx, a.sx), k = unflip(fk, ny, a.sy);
-                const long g = (long) (k - 1)*nz*nx + (long) (j - 1)*nz
-                             + ((a.sz > 0) ? fi : nz - 1 - fi);
-                const R v = ring[t & (RING - 1)][(lk + 1)*(BX + 1) + lj + 1];
-                if (colB && fi >= a.fz0) a.tt[g] = v;
+            #pragma unroll
+            for (int c = 0; c < NW; c++) {
+                const int fi = t - sdep[c];
+                if (!(sflag[c] & 1) || fi < ez0 || fi > a.fz1) continue;
+                const bool upd = fi >= a.fz0, post = (sflag[c] & 2) != 0;
+                if (!upd && !post) continue;
+                const long g = sbase[c] + ((a.sz > 0) ? fi : nz - 1 - fi);
+                const R v = ring[t & (RING - 1)][sring[c]];
+                if (upd) {
+                    if (CHAIN) st_coh(a.tt + g, v);
+                    else a.tt[g] = v;
+                }
                 if (post) mb_put(a.mb, g, v, tag);
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if (sl == 0)
                 __hip_atomic_store(&s_sdone, t, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (CHAIN && (((t - tlo) & 3) == 3 || t == thi)) {
+                // this wave's field and mailbox stores are device-coherent
+                // (written through the L2): once they have completed, the
+                // progress word may announce them.  No cache write-back or
+                // invalidation is involved on either side.
+                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                if (sl == 0)
+                    __hip_atomic_store(a.prog + (long) a.par*a.nbx*a.nby + wg,
+                                       (a.serial << 32) |
+                                       (t == thi ? kDone : (unsigned) t),
+                                       __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         return;
     }
@@ -536,28 +641,110 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         // polled one step at a time.
         constexpr int HD = 8;
         const int s_beg = tlo - 3, s_end = thi - 1;
+        // Sweep chains: lane l < 9 watches one tile of the preceding sweep
+        // (see above).  Invariant: once step s is released, the preceding
+        // sweep is done with every node this tile touches through step
+        // s + 2*HD (the compute waves load their columns up to 2*HB steps
+        // ahead, the direct halo values up to 2*HD).
+        int ptile = -1;
+        unsigned long long seen = 0;
+        int verified = s_beg - 1;
+        if (CHAIN && a.pserial) {
+            auto trange = [&](int f0, int bw, bool same, int n, int b0,
+                              int nb, int& t0, int& t1) {
+                int lo = same ? f0 - 1 : n - 1 - f0 - bw;
+                int hi = same ? f0 + bw : n - f0;
+                lo = min(max(lo, b0), n - 1);
+                hi = min(max(hi, b0), n - 1);
+                t0 = min((lo - b0)/bw, nb - 1);
+                t1 = min((hi - b0)/bw, nb - 1);
+            };
+            int tx0, tx1, ty0, ty1;
+            trange(fj0, BX, a.psx == a.sx, nx, a.fx0, a.nbx, tx0, tx1);
+            trange(fk0, BY, a.psy == a.sy, ny, a.fy0, a.nby, ty0, ty1);
+            const int ntx = tx1 - tx0 + 1;
+            if (hl < ntx*(ty1 - ty0 + 1))
+                ptile = (tx0 + hl % ntx)*a.nby + ty0 + hl / ntx;
+        }
+        auto verify = [&](int T) {
+            if (!CHAIN || !a.pserial || T <= verified) return;
+            int need;
+            if (a.psz == a.sz) {
+                need = T + 3;
+                if (a.psx != a.sx) need += nx + 1 - 2*fj0;
+                if (a.psy != a.sy) need += ny + 1 - 2*fk0;
+            } else {
+                need = nz - s_beg + 3;
+                need += (a.psx == a.sx) ? 2*(fj0 + BX) : nx - 1;
+                need += (a.psy == a.sy) ? 2*(fk0 + BY) : ny - 1;
+            }
+            need = min(max(need, 0), (int) kDone);
+            const unsigned long long req = (a.pserial << 32) | (unsigned) need;
+            if (ptile >= 0) {
+                const unsigned long long* pw =
+                    a.prog + (long) (1 - a.par)*a.nbx*a.nby + ptile;
+                unsigned long long spins = 0;
+                while (seen < req) {
+                    seen = __hip_atomic_load(pw, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                    if (seen >= req) break;
+                    __builtin_amdgcn_s_sleep(2);
+                    // another workgroup gave up: stop waiting for it
+                    if ((++spins & 1023) == 0 &&
+                        __hip_atomic_load(a.err, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                    if (spins > kSpinLimit) {
+                        atomicExch(a.err, 1);
+                        __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                }
+            }
+            // the loads this releases are device-coherent and issued after
+            // the progress words were read
+            verified = T;
+        };
         R cur[HD], nxt[HD];
         auto fetch = [&](int s0, R* buf) {
             #pragma unroll
             for (int q = 0; q < HD; q++) {
                 const int fi = s0 + q - hj - hk;
                 buf[q] = R(0);
-                if (direct && hcol && fi >= ez0 && fi <= a.fz1)
-                    buf[q] = a.tt[gcol + ((a.sz > 0) ? fi : nz - 1 - fi)];
+                if (direct && hcol && fi >= ez0 && fi <= a.fz1) {
+                    const R* p = a.tt + gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
+                    buf[q] = CHAIN ? ld_coh(p) : *p;
+                }
             }
         };
+        // after a wait-limit failure elsewhere the tile gives up at once
+        if (CHAIN && __hip_atomic_load(a.err, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+        verify(s_beg + 3*HD - 1);
+        if (CHAIN && lds_load(&s_abort)) return;
         fetch(s_beg, cur);
         for (int s0 = s_beg; s0 <= s_end; s0 += HD) {
+            verify(s0 + 3*HD - 1);
+            if (CHAIN && lds_load(&s_abort)) return;
             if (s0 + HD <= s_end) fetch(s0 + HD, nxt);
             #pragma unroll
             for (int q = 0; q < HD; q++) {
                 const int s = s0 + q;
                 if (s > s_end) break;                 // wave-uniform
                 // the ring slot of step s was last read at step s - RING + 3
-                const int need = NW*(s - RING + 3 - tlo + 1);
+                const int need = s - RING + 3;
                 unsigned long long spins = 0;
-                while (lds_load(&s_cdone) < need) {
+                while (wdone_min() < need) {
                     if (++spins > kSpinLimit || lds_load(&s_abort)) return;
+                    __builtin_amdgcn_s_sleep(1);
                 }
                 const int fi = s - hj - hk;
                 if (hcol && fi >= ez0 && fi <= a.fz1) {
@@ -591,6 +778,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     }
 
     // -------------------------------------------------------- compute waves
+    const int wv = threadIdx.x / 64;
     const int lj = threadIdx.x % BX, lk = threadIdx.x / BX;
     const int fj = fj0 + lj, fk = fk0 + lk;
     const bool colE = (fj >= jlo && fj <= jhi && fk >= klo && fk <= khi);
@@ -626,7 +814,9 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     auto prefetch = [&](int tb) {
         const int f0 = tb - fdep;
         #pragma unroll
-        for (int q = 0; q < HB; q++) tn[q] = col[node_off(f0 + q)];
+        for (int q = 0; q < HB; q++)
+            tn[q] = CHAIN ? ld_coh(col + node_off(f0 + q))
+                          : col[node_off(f0 + q)];
         #pragma unroll
         for (int q = 0; q <= HB; q++) {
             const int z = layer_off(f0 + q);
@@ -634,6 +824,20 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
         }
     };
+    if (CHAIN) {
+        // the halo wave's release of step tlo - 3 covers the loads below
+        unsigned long long spins = 0;
+        while (lds_load(&s_hready) < tlo - 2) {
+            if (lds_load(&s_abort) || ++spins > kSpinLimit) {
+                atomicExch(a.err, 1);
+                __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                return;
+            }
+            // waits for the preceding sweep: leave the SIMD to others
+            __builtin_amdgcn_s_sleep(8);
+        }
+    }
     if (!INIT) prefetch(tlo);
     for (int tb = tlo; tb <= thi; tb += HB) {
         if (INIT) prefetch(tb);
@@ -650,11 +854,14 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         for (int q = 0; q < HB; q++) {
             const int t = tb + q;
             if (t > thi) break;                       // workgroup-uniform
-            // halo of steps < t, every compute wave past step t-1, and the
-            // store wave done with the ring slot this step overwrites
-            const int needC = NW*(t - tlo);
+            // halo of steps < t, the wave above (w - 1) past step t - 1, and
+            // the readers of the ring slot this step overwrites (wave w + 1,
+            // the store wave) past step t - RING + 3
             unsigned long long spins = 0;
-            while (lds_load(&s_hready) < t || lds_load(&s_cdone) < needC ||
+            while (lds_load(&s_hready) < t ||
+                   (wv > 0 && lds_load(&s_wdone[wv - 1]) < t - 1) ||
+                   (wv < NW - 1 &&
+                    lds_load(&s_wdone[wv + 1]) < t - RING + 3) ||
                    (SW && lds_load(&s_sdone) < t - RING + 3)) {
                 if (lds_load(&s_abort) || ++spins > kSpinLimit) {
                     atomicExch(a.err, 1);
@@ -662,6 +869,9 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                     return;
                 }
+                // a waiting wave yields its issue slots to the waves of
+                // other tiles sharing the SIMD
+                __builtin_amdgcn_s_sleep(1);
             }
             const int fi = t - fj - fk;
             if (colE && fi >= ez0 && fi <= a.fz1) {
@@ -696,8 +906,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if ((threadIdx.x & 63) == 0)
-                __hip_atomic_fetch_add(&s_cdone, 1, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&s_wdone[wv], t, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     if (a.maxdelta && dmax > R(0)) {

@@ -112,6 +112,23 @@ void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
  * reproduce the tile's values exactly, so results are unchanged (on by
  * default; mailbox tile shapes only). */
 void fteik_solver3d_setTileSkipping(int on, int *ierr);
+/* EXTENSION: run the Gauss-Seidel iteration sweeps of a solve as one chain
+ * in which each sweep starts while the one before it is still running,
+ * wherever the nodes it reads and writes are already final in the one
+ * before (results are bitwise those of one sweep after another).  on = 1:
+ * chains on grids whose sweep has at most one tile per CU, where a lone
+ * sweep leaves CUs idle; 2: on every grid; 0 (default): never.  A chain
+ * does every update of every sweep, so it pays only while most tiles still
+ * change; tile skipping does not apply inside a chain.  getSweepOverlap reports whether FP64
+ * solves of this grid chain their sweeps and the tile shapes of the FP64
+ * and FP32 chains (-1: none). */
+void fteik_solver3d_setSweepOverlap(int on, int *ierr);
+/* EXTENSION: tile shapes of the FP64 and FP32 sweep chains (store-wave
+ * mailbox shapes whose two sweeps' tiles fit the GPU; chosen by the library
+ * at initialize). */
+void fteik_solver3d_setChainTile(int tile, int tile32, int *ierr);
+void fteik_solver3d_getSweepOverlap(int *on, int *tile, int *tile32,
+                                    int *ierr);
 /* EXTENSION: device time of the last solve (ms, hipEvent), number of FP64
  * correction iterations run (mixed mode) and node updates performed. */
 void fteik_solver3d_getLastSolveInfo(double *ms, int *corrIterations,

@@ -264,8 +264,9 @@ def test_every_tile_shape_matches_oracle(precision):
     g.set_sources(zs, xs, ys)
     L = lib()
     t, t32, nt, ierr = c_int(), c_int(), c_int(), c_int()
+    L.fteik_solver3d_setSweepOverlap(0, byref(ierr))   # one launch per sweep
     L.fteik_solver3d_getTile(byref(t), byref(t32), byref(nt), byref(ierr))
-    assert ierr.value == 0 and nt.value >= 13
+    assert ierr.value == 0 and nt.value >= 14
     tested = 0
     for tile in range(nt.value):
         L.fteik_solver3d_setTile(tile, tile, byref(ierr))
@@ -290,6 +291,7 @@ def test_tile_skipping_is_exact():
     fields = []
     for on in (1, 0):
         g = GPU3D(nz, nx, ny, h, nsweep=8)
+        L.fteik_solver3d_setSweepOverlap(0, byref(ierr))
         L.fteik_solver3d_setTileSkipping(on, byref(ierr))
         g.set_cells(vel)
         g.set_sources(zs, xs, ys)
@@ -297,3 +299,42 @@ def test_tile_skipping_is_exact():
         fields.append(g.field())
     np.testing.assert_array_equal(fields[0], ref)
     np.testing.assert_array_equal(fields[1], ref)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("case", [
+    # nz, nx, ny, h, source (z, x, y) in cells, nsweep
+    (40, 37, 33, 12.0, (30.25, 3.5, 20.0), 3),
+    (33, 64, 49, 10.0, (1.5, 60.2, 2.7), 2),
+    (71, 20, 45, 20.0, (35.0, 10.0, 22.0), 5),
+])
+def test_sweep_overlap_matches_oracle(case, precision):
+    """Iteration sweeps chained in one launch, each overlapping the one
+    before it, for every store-wave tile shape: the field is the oracle's
+    bit for bit (FP64) and the same as one launch per sweep (mixed)."""
+    nz, nx, ny, h, (zc, xc, yc), nsweep = case
+    vel = hetero(nz, nx, ny, h, 5)
+    zs, xs, ys = zc * h, xc * h, yc * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, nsweep, 5.0)
+    L = lib()
+    ierr = c_int()
+    g = GPU3D(nz, nx, ny, h, nsweep=nsweep, precision=precision)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    on, t, t32 = c_int(), c_int(), c_int()
+    L.fteik_solver3d_setSweepOverlap(1, byref(ierr))   # small grid: chained
+    L.fteik_solver3d_getSweepOverlap(byref(on), byref(t), byref(t32),
+                                     byref(ierr))
+    assert ierr.value == 0 and on.value == 1 and t.value >= 0
+    L.fteik_solver3d_setSweepOverlap(0, byref(ierr))
+    assert g.solve(1) == 0
+    plain = g.field()
+    if precision == 0:
+        np.testing.assert_array_equal(plain, ref)
+    L.fteik_solver3d_setSweepOverlap(2, byref(ierr))
+    for tile in (10, 11, 12, 13):
+        L.fteik_solver3d_setChainTile(tile, tile, byref(ierr))
+        assert ierr.value == 0
+        _announce("chain tile", tile, "precision", precision, "case", case)
+        assert g.solve(1) == 0
+        np.testing.assert_array_equal(g.field(), plain)
