@@ -309,10 +309,13 @@ def main():
     vel, (zs, xs, ys) = layered_model()
     ierr = c_int(1)
 
-    def init(precision, nsweep):
+    def init(precision, nsweep, skipping=0, overlap=0):
         L.fteik_solver3d_initialize(N, N, N, 0.0, 0.0, 0.0, H, H, H, nsweep,
                                     5.0, 0.0, 0, byref(ierr))
         assert ierr.value == 0, "initialize"
+        L.fteik_solver3d_setTileSkipping(skipping, byref(ierr))
+        L.fteik_solver3d_setSweepOverlap(overlap, byref(ierr))
+        assert ierr.value == 0, "sweep options"
         L.fteik_solver3d_setVelocityModel64f(vel.size, vel.ctypes.data_as(
             ctypes.POINTER(c_double)), byref(ierr))
         assert ierr.value == 0, "model"
@@ -378,6 +381,7 @@ def main():
           file=sys.stderr, flush=True)
 
     # ---------------------------------------------------------- FP64 path
+    # Headline: every update of every sweep performed (no tile skipping).
     init(0, nsweep)
     t64 = timed(args.steps, args.warmup)
     _, _, upd64 = info()
@@ -386,8 +390,22 @@ def main():
           flush=True)
     ms_launch64 = sweep_ms(0)
 
+    # Same solve with exact tile skipping (the library default) and with
+    # chained sweeps: time to the same field, not an update rate.
+    init(0, nsweep, skipping=1)
+    tskip = timed(args.steps, args.warmup)
+    skip_equal = bool(np.array_equal(field(), tt64))
+    init(0, nsweep, overlap=2)
+    tchain = timed(args.steps, args.warmup)
+    chain_equal = bool(np.array_equal(field(), tt64))
+    print("bench: fp64 with tile skipping %.3f ms (equal %s), chained "
+          "sweeps %.3f ms (equal %s)" % (tskip * 1e3, skip_equal,
+                                         tchain * 1e3, chain_equal),
+          file=sys.stderr, flush=True)
+
     # --------------------------------------------------------- mixed path
-    init(1, nsweep)
+    # library defaults for the FP64 corrections (tile skipping on)
+    init(1, nsweep, skipping=1)
     tmix = timed(args.steps, args.warmup)
     _, corr, updmix = info()
     ttm = field()
@@ -443,9 +461,20 @@ def main():
                    "grid": [N, N, N], "h": H, "nsweep": nsweep, "eps": 5.0,
                    "source_zxy": [zs, xs, ys],
                    "updates_per_solve": upd64},
+        "fp64_tile_skipping": {
+            "ms_per_solve": tskip * 1e3, "speedup_vs_dense": t64 / tskip,
+            "field_equals_dense": skip_equal,
+            "note": "same solve, tiles whose pass would reproduce their "
+                    "values exactly are skipped; time to the same field"},
+        "fp64_chained_sweeps": {
+            "ms_per_solve": tchain * 1e3, "speedup_vs_dense": t64 / tchain,
+            "field_equals_dense": chain_equal,
+            "note": "all 8*nsweep iteration sweeps in one launch, each "
+                    "overlapping the one before it; every update performed"},
         "mixed": {"value": upd64 / tmix / 1e9, "unit": "Gupdates/s",
                   "ms_per_step": tmix * 1e3, "speedup_vs_fp64": t64 / tmix,
                   "fp32_iterations": nsweep,
+                  "speedup_vs_fp64_tile_skipping": tskip / tmix,
                   "fp64_correction_iterations": corr,
                   "updates_performed": updmix,
                   "max_rel_err_vs_fp64": float(rel.max()),
