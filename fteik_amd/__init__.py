@@ -137,6 +137,7 @@ def load_library(path=None):
         "fteik_batch3d_setGrid": [i, i, i, i, d, d, d, d, d, d, _dp, d, d, d,
                                   _ip],
         "fteik_batch3d_setPrecision": [i, _ip],
+        "fteik_batch3d_setMixedParameters": [d, i, _ip],
         "fteik_batch3d_setTile": [i, _ip],
         "fteik_batch3d_solve": [i, d, _ip],
         "fteik_batch3d_getLastSolveInfo": [_dp, _ip, _dp, _ip],

@@ -242,7 +242,7 @@ struct Solver3D {
     int precision = 0, schedule = 0, cooperative = 1;
     int tile = 0, tile32 = 0;            // pipelined-sweep tile shapes
     std::vector<char> tileOK;            // tile fits co-resident (both types)
-    double mixedBias = 1.0/256.0;        // 2^-8 relative promotion margin
+    double mixedBias = 1.0/64.0;         // 2^-6 relative promotion margin
     int maxCorr = 64;
     std::vector<SrcInfo> src;
     std::vector<int> ri;
@@ -835,9 +835,9 @@ void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
                                        int *ierr)
 {
     *ierr = 0;
-    if (!(bias >= 0.0 && bias < 1.0e-2) || maxCorrections < 1 ||
+    if (!(bias >= 0.0 && bias < 0.1) || maxCorrections < 1 ||
         maxCorrections > 64) {
-        msg("fteik_solver3d_setMixedParameters: need 0 <= bias < 1e-2 and "
+        msg("fteik_solver3d_setMixedParameters: need 0 <= bias < 0.1 and "
             "1 <= maxCorrections <= 64");
         *ierr = 1;
         return;
@@ -2427,7 +2427,7 @@ struct BGrid {
 struct Batch3D {
     bool init = false, haveTimes = false;
     int precision = 0, tile = 6, maxCorr = 16;
-    double bias = 1.0/256.0, convTol = 0.0;
+    double bias = 1.0/64.0, convTol = 0.0;
     std::vector<BGrid> g;
     fteik::BatchGrid *d_desc = nullptr;
     int *d_map = nullptr, nwg = 0, mapTile = -1;
@@ -2611,6 +2611,21 @@ void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
     }
     x.set = true;
     gb.mapTile = -1;
+}
+
+void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
+                                      int *ierr)
+{
+    *ierr = 0;
+    if (!(bias >= 0.0 && bias < 0.1) || maxCorrections < 1 ||
+        maxCorrections > 64) {
+        msg("fteik_batch3d_setMixedParameters: need 0 <= bias < 0.1 and "
+            "1 <= maxCorrections <= 64");
+        *ierr = 1;
+        return;
+    }
+    gb.bias = bias;
+    gb.maxCorr = maxCorrections;
 }
 
 void fteik_batch3d_setPrecision(int mode, int *ierr)

@@ -105,9 +105,10 @@ void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
         // the plan lists the processed tiles; the other workgroups exit
         if ((int) blockIdx.x >= a.plan[0]) return;
         pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(
-            a, a.plan[1 + blockIdx.x]);
+            a, dyn_of(a), a.plan[1 + blockIdx.x]);
     } else {
-        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(a, blockIdx.x);
+        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP>(a, dyn_of(a),
+                                                        blockIdx.x);
     }
 }
 
@@ -129,16 +130,8 @@ void sweep3d_chain_kernel(SweepArgs<R> a, int dir0)
     const int ntile = a.nbx*a.nby;
     const int q = (int) blockIdx.x / ntile;
     const int wg = (int) blockIdx.x - q*ntile;
-    SweepArgs<R> b = a;
-    sweep_signs((dir0 + q) % 8 + 1, b.sz, b.sx, b.sy);
-    b.serial = a.serial + q;
-    b.par = q & 1;
-    b.pserial = 0;
-    if (q > 0) {
-        sweep_signs((dir0 + q - 1) % 8 + 1, b.psz, b.psx, b.psy);
-        b.pserial = a.serial + q - 1;
-    }
-    pencil_mb_body<R, false, BX, BY, HB, 2, false, true>(b, wg);
+    pencil_mb_body<R, false, BX, BY, HB, 2, false, true>(
+        a, chain_dyn(dir0, q, a.serial), wg);
 }
 
 // Tile-skipping plan of one iteration sweep (one workgroup).  A tile whose

@@ -114,6 +114,26 @@ struct SweepArgs {
     int psz, psx, psy;
 };
 
+// The per-sweep values of a mailbox sweep, kept apart from SweepArgs so a
+// kernel that runs several sweeps (chains) need not copy SweepArgs.
+struct SweepDyn {
+    int sz, sx, sy;                  // time signs of the sweep
+    int psz, psx, psy;               // those of the preceding sweep (chains)
+    int par;                         // progress-word set (chains)
+    unsigned long long serial;       // launch serial / mailbox tag
+    unsigned long long pserial;      // preceding sweep's serial (0: none)
+};
+
+template <typename R>
+__device__ __forceinline__ SweepDyn dyn_of(const SweepArgs<R>& a)
+{
+    SweepDyn d;
+    d.sz = a.sz; d.sx = a.sx; d.sy = a.sy;
+    d.psz = a.psz; d.psx = a.psx; d.psy = a.psy;
+    d.par = a.par; d.serial = a.serial; d.pserial = a.pserial;
+    return d;
+}
+
 constexpr unsigned kDone = 0x7fffffffu;
 // s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for
 constexpr int kWaitVmcnt0 = 0x0f70;
@@ -506,6 +526,7 @@ __device__ __forceinline__ int lds_load(int* p)
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE,
           bool SKIP, bool CHAIN = false>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
+                                               const SweepDyn& d,
                                                const int wg)
 {
     static_assert(!(INIT && SKIP), "tile skipping is for iteration sweeps");
@@ -538,7 +559,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     if (jlo > jhi || klo > khi) return;
     const int tlo = ez0 + jlo + klo;
     const int thi = a.fz1 + jhi + khi;
-    const unsigned tag = (unsigned) a.serial;
+    const unsigned tag = (unsigned) d.serial;
     if (threadIdx.x == 0) {
         for (int w = 0; w < NW; w++) s_wdone[w] = tlo - 1;
         s_hready = tlo - 3; s_sdone = tlo - 1; s_abort = 0;
@@ -569,8 +590,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             sflag[c] = colE ? (1 | (post ? 2 : 0)) : 0;
             sdep[c] = fj + fk;
             sring[c] = (lk + 1)*(BX + 1) + lj + 1;
-            sbase[c] = colE ? (long) (unflip(fk, ny, a.sy) - 1)*nz*nx
-                              + (long) (unflip(fj, nx, a.sx) - 1)*nz : 0;
+            sbase[c] = colE ? (long) (unflip(fk, ny, d.sy) - 1)*nz*nx
+                              + (long) (unflip(fj, nx, d.sx) - 1)*nz : 0;
         }
         for (int t = tlo; t <= thi; t++) {
             unsigned long long spins = 0;
@@ -584,7 +605,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 if (!(sflag[c] & 1) || fi < ez0 || fi > a.fz1) continue;
                 const bool upd = fi >= a.fz0, post = (sflag[c] & 2) != 0;
                 if (!upd && !post) continue;
-                const long g = sbase[c] + ((a.sz > 0) ? fi : nz - 1 - fi);
+                const long g = sbase[c] + ((d.sz > 0) ? fi : nz - 1 - fi);
                 const R v = ring[t & (RING - 1)][sring[c]];
                 if (upd) {
                     if (CHAIN) st_coh(a.tt + g, v);
@@ -603,8 +624,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 // invalidation is involved on either side.
                 __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
                 if (sl == 0)
-                    __hip_atomic_store(a.prog + (long) a.par*a.nbx*a.nby + wg,
-                                       (a.serial << 32) |
+                    __hip_atomic_store(a.prog + (long) d.par*a.nbx*a.nby + wg,
+                                       (d.serial << 32) |
                                        (t == thi ? kDone : (unsigned) t),
                                        __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -625,8 +646,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         // halo columns inside the extended box (one column row before it)
         const bool hcol = hl < HCOLS && hj >= ex0 && hk >= ey0 &&
                           hj <= a.fx1 && hk <= a.fy1;
-        const long gcol = hcol ? (long) (unflip(hk, ny, a.sy) - 1)*nz*nx
-                               + (long) (unflip(hj, nx, a.sx) - 1)*nz : 0;
+        const long gcol = hcol ? (long) (unflip(hk, ny, d.sy) - 1)*nz*nx
+                               + (long) (unflip(hj, nx, d.sx) - 1)*nz : 0;
         // columns outside the box, and those of a skipped upwind tile, are
         // not written in this launch: they are read from the field instead
         // of the mailbox
@@ -649,7 +670,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         int ptile = -1;
         unsigned long long seen = 0;
         int verified = s_beg - 1;
-        if (CHAIN && a.pserial) {
+        if (CHAIN && d.pserial) {
             auto trange = [&](int f0, int bw, bool same, int n, int b0,
                               int nb, int& t0, int& t1) {
                 int lo = same ? f0 - 1 : n - 1 - f0 - bw;
@@ -660,29 +681,29 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 t1 = min((hi - b0)/bw, nb - 1);
             };
             int tx0, tx1, ty0, ty1;
-            trange(fj0, BX, a.psx == a.sx, nx, a.fx0, a.nbx, tx0, tx1);
-            trange(fk0, BY, a.psy == a.sy, ny, a.fy0, a.nby, ty0, ty1);
+            trange(fj0, BX, d.psx == d.sx, nx, a.fx0, a.nbx, tx0, tx1);
+            trange(fk0, BY, d.psy == d.sy, ny, a.fy0, a.nby, ty0, ty1);
             const int ntx = tx1 - tx0 + 1;
             if (hl < ntx*(ty1 - ty0 + 1))
                 ptile = (tx0 + hl % ntx)*a.nby + ty0 + hl / ntx;
         }
         auto verify = [&](int T) {
-            if (!CHAIN || !a.pserial || T <= verified) return;
+            if (!CHAIN || !d.pserial || T <= verified) return;
             int need;
-            if (a.psz == a.sz) {
+            if (d.psz == d.sz) {
                 need = T + 3;
-                if (a.psx != a.sx) need += nx + 1 - 2*fj0;
-                if (a.psy != a.sy) need += ny + 1 - 2*fk0;
+                if (d.psx != d.sx) need += nx + 1 - 2*fj0;
+                if (d.psy != d.sy) need += ny + 1 - 2*fk0;
             } else {
                 need = nz - s_beg + 3;
-                need += (a.psx == a.sx) ? 2*(fj0 + BX) : nx - 1;
-                need += (a.psy == a.sy) ? 2*(fk0 + BY) : ny - 1;
+                need += (d.psx == d.sx) ? 2*(fj0 + BX) : nx - 1;
+                need += (d.psy == d.sy) ? 2*(fk0 + BY) : ny - 1;
             }
             need = min(max(need, 0), (int) kDone);
-            const unsigned long long req = (a.pserial << 32) | (unsigned) need;
+            const unsigned long long req = (d.pserial << 32) | (unsigned) need;
             if (ptile >= 0) {
                 const unsigned long long* pw =
-                    a.prog + (long) (1 - a.par)*a.nbx*a.nby + ptile;
+                    a.prog + (long) (1 - d.par)*a.nbx*a.nby + ptile;
                 unsigned long long spins = 0;
                 while (seen < req) {
                     seen = __hip_atomic_load(pw, __ATOMIC_RELAXED,
@@ -716,7 +737,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 const int fi = s0 + q - hj - hk;
                 buf[q] = R(0);
                 if (direct && hcol && fi >= ez0 && fi <= a.fz1) {
-                    const R* p = a.tt + gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
+                    const R* p = a.tt + gcol + ((d.sz > 0) ? fi : nz - 1 - fi);
                     buf[q] = CHAIN ? ld_coh(p) : *p;
                 }
             }
@@ -750,7 +771,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 if (hcol && fi >= ez0 && fi <= a.fz1) {
                     R v = cur[q];
                     if (!direct) {
-                        const long g = gcol + ((a.sz > 0) ? fi : nz - 1 - fi);
+                        const long g = gcol + ((d.sz > 0) ? fi : nz - 1 - fi);
                         spins = 0;
                         while (!mb_get(a.mb, g, tag, v)) {
                             __builtin_amdgcn_s_sleep(1);
@@ -785,13 +806,13 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     const bool colB = colE && fj >= a.fx0 && fk >= a.fy0;
     const bool post = colE && (lj == BX - 1 || lk == BY - 1);
     const int cj = min(max(fj, jlo), jhi), ck = min(max(fk, klo), khi);
-    const int j = unflip(cj, nx, a.sx), k = unflip(ck, ny, a.sy);
+    const int j = unflip(cj, nx, d.sx), k = unflip(ck, ny, d.sy);
     const long colbase = (long) (k - 1)*nz*nx + (long) (j - 1)*nz;
     const R* const col = a.tt + colbase;
     const int jm = max(j - 1, 1), jp = min(j, nx - 1);
     const int km = max(k - 1, 1), kp = min(k, ny - 1);
-    const int xu = (a.sx > 0) ? jm : jp, xd = (a.sx > 0) ? jp : jm;
-    const int yu = (a.sy > 0) ? km : kp, yd = (a.sy > 0) ? kp : km;
+    const int xu = (d.sx > 0) ? jm : jp, xd = (d.sx > 0) ? jp : jm;
+    const int yu = (d.sy > 0) ? km : kp, yd = (d.sy > 0) ? kp : km;
     const long szx = (long) (nz - 1)*(nx - 1);
     const R* const c_uu = a.slow + (long) (yu - 1)*szx + (long) (xu - 1)*(nz - 1);
     const R* const c_ud = a.slow + (long) (yd - 1)*szx + (long) (xu - 1)*(nz - 1);
@@ -804,10 +825,10 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     R dmax = R(0);
     auto node_off = [&](int fi) -> int {
         fi = min(max(fi, ez0), a.fz1);
-        return (a.sz > 0) ? fi : nz - 1 - fi;
+        return (d.sz > 0) ? fi : nz - 1 - fi;
     };
     auto layer_off = [&](int f) -> int {
-        const int zc = (a.sz > 0) ? f : nz - f;
+        const int zc = (d.sz > 0) ? f : nz - f;
         return min(max(zc, 1), nz - 1) - 1;
     };
     R tn[HB], cn[HB + 1][4];
@@ -892,8 +913,8 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                     slowness7_ud<R>(cc[q], cc[q + 1], sl);
                     R vn;
                     if (INIT) {
-                        const int i = unflip(fi, nz, a.sz);
-                        vn = init3d(c, nb, sl, v, i, j, k, a.sz, a.sx, a.sy);
+                        const int i = unflip(fi, nz, d.sz);
+                        vn = init3d(c, nb, sl, v, i, j, k, d.sz, d.sx, d.sy);
                     } else {
                         vn = noinit3d(c, nb, sl, v);
                     }
@@ -916,7 +937,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                   (unsigned long long) __double_as_longlong(dd));
     }
     if (SKIP && colB && dmax > R(0))
-        a.colmod[(long) (k - 1)*nx + (j - 1)] = a.serial;
+        a.colmod[(long) (k - 1)*nx + (j - 1)] = d.serial;
 }
 
 __host__ __device__ inline void sweep_signs(int sweep, int& sz, int& sx,
@@ -930,6 +951,24 @@ __host__ __device__ inline void sweep_signs(int sweep, int& sz, int& sx,
 }
 
 // solver3D.F90:493-565 box, converted to the flipped 0-based frame.
+// Per-sweep values of sweep q of a chain starting in direction dir0 + 1
+// with serial serial0.
+__device__ __forceinline__ SweepDyn chain_dyn(int dir0, int q,
+                                              unsigned long long serial0)
+{
+    SweepDyn d;
+    sweep_signs((dir0 + q) % 8 + 1, d.sz, d.sx, d.sy);
+    d.serial = serial0 + q;
+    d.par = q & 1;
+    d.pserial = 0;
+    d.psz = d.psx = d.psy = 0;
+    if (q > 0) {
+        sweep_signs((dir0 + q - 1) % 8 + 1, d.psz, d.psx, d.psy);
+        d.pserial = serial0 + q - 1;
+    }
+    return d;
+}
+
 __host__ __device__ inline void flipped_box(int sweep, bool linit, int nz,
                                             int nx, int ny, int zsi, int xsi,
                                             int ysi, int* b)

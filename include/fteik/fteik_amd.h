@@ -119,9 +119,9 @@ void fteik_solver3d_setTileSkipping(int on, int *ierr);
  * chains on grids whose sweep has at most one tile per CU, where a lone
  * sweep leaves CUs idle; 2: on every grid; 0 (default): never.  A chain
  * does every update of every sweep, so it pays only while most tiles still
- * change; tile skipping does not apply inside a chain.  getSweepOverlap reports whether FP64
- * solves of this grid chain their sweeps and the tile shapes of the FP64
- * and FP32 chains (-1: none). */
+ * change; tile skipping does not apply inside a chain.  getSweepOverlap
+ * reports whether FP64 solves of this grid chain their sweeps and the tile
+ * shapes of the FP64 and FP32 chains (-1: none). */
 void fteik_solver3d_setSweepOverlap(int on, int *ierr);
 /* EXTENSION: tile shapes of the FP64 and FP32 sweep chains (store-wave
  * mailbox shapes whose two sweeps' tiles fit the GPU; chosen by the library
@@ -236,6 +236,10 @@ void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
                            double dy, const double *vel, double zs, double xs,
                            double ys, int *ierr);
 void fteik_batch3d_setPrecision(int mode, int *ierr);
+/* promotion margin and correction cap of the batch's mixed mode (see
+ * fteik_solver3d_setMixedParameters) */
+void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
+                                      int *ierr);
 /* tile shape for the whole batch (see fteik_solver3d_setTile) */
 void fteik_batch3d_setTile(int tile, int *ierr);
 void fteik_batch3d_solve(int nsweep, double eps, int *ierr);
