@@ -297,6 +297,9 @@ def main():
                     help="skip the batched-solve and locator sections")
     ap.add_argument("--sur-steps", type=int, default=3000,
                     help="training steps of the locator surrogate")
+    ap.add_argument("--only-locator", action="store_true",
+                    help="run only the locator and surrogate sections "
+                         "(for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
@@ -306,6 +309,9 @@ def main():
 
     import fteik_amd
     L = fteik_amd.load_library()
+    if args.only_locator:
+        print(json.dumps({"locator": locator_section(L, args)}), flush=True)
+        return
     vel, (zs, xs, ys) = layered_model()
     ierr = c_int(1)
 

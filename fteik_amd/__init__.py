@@ -156,6 +156,9 @@ def load_library(path=None):
         "locate_locateEventsSurrogate": [i, _ip, i, i, _ip, _dp, _dp, _ip,
                                          _ip],
         "fteik_amd_gemmBf16": [i, i, i, _fp, _fp, _fp, _ip],
+        "fteik_amd_invHiddenBf16": [i, _fp, _fp, _fp,
+                                    ctypes.POINTER(ctypes.c_ushort),
+                                    ctypes.POINTER(ctypes.c_ushort), _ip],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

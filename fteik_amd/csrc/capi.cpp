@@ -1864,7 +1864,20 @@ void locate_setReferenceCompatibility(int mode, int *ierr)
 }
 
 // float sum in index order (locate.F90:654 SUM over the padded weights)
-static float weight_norm(const float *w, int lntf)
+static // Per-event arrays for the batched grid search: observations and weights
+// of skipped picks zeroed (their terms then add +-0), and wt2 = sqrt(1/2) *
+// wt, the product the reference forms per pick (locate.F90:694-706).
+void mask_skipped(float *o, float *w, float *w2, const unsigned char *sk,
+                  size_t n)
+{
+    const float sqrtHalf = 0.70710677f;   // SQRT(0.5) in single precision
+    for (size_t i = 0; i < n; i++) {
+        if (sk[i]) { o[i] = 0.0f; w[i] = 0.0f; }
+        w2[i] = sqrtHalf*w[i];
+    }
+}
+
+float weight_norm(const float *w, int lntf)
 {
     float s = 0.0f;
     for (int i = 0; i < lntf; i++) s = s + w[i];
@@ -1945,22 +1958,26 @@ void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
         memcpy(&sk[(size_t) e*gl.lntf], &gl.skip[k], (size_t) gl.lntf);
         xn[(size_t) e] = weight_norm(&gl.wt[k], gl.lntf);
     }
-    float *d_o = nullptr, *d_w = nullptr, *d_xn = nullptr;
+    std::vector<float> w2(w.size());
+    mask_skipped(o.data(), w.data(), w2.data(), sk.data(), o.size());
+    float *d_o = nullptr, *d_w = nullptr, *d_w2 = nullptr, *d_xn = nullptr;
     unsigned char *d_sk = nullptr;
     unsigned long long *d_best = nullptr;
     std::vector<unsigned long long> best((size_t) n, ~0ull);
     bool ok = dalloc(&d_o, o.size()) && dalloc(&d_w, w.size()) &&
+              dalloc(&d_w2, w2.size()) &&
               dalloc(&d_xn, xn.size()) && dalloc(&d_sk, sk.size()) &&
               dalloc(&d_best, best.size()) &&
               h2d(d_o, o.data(), o.size()*sizeof(float)) &&
               h2d(d_w, w.data(), w.size()*sizeof(float)) &&
+              h2d(d_w2, w2.data(), w2.size()*sizeof(float)) &&
               h2d(d_xn, xn.data(), xn.size()*sizeof(float)) &&
               h2d(d_sk, sk.data(), sk.size()) &&
               h2d(d_best, best.data(), best.size()*sizeof(unsigned long long));
     if (ok) {
         int rc = fteik_k_locate_batch(gl.d_tab, gl.ntf, gl.ngrd, gl.ldgrd, d_o,
-                                      d_w, d_sk, d_xn, gl.lntf, n, d_best,
-                                      nullptr, g_dev.stream);
+                                      d_w, d_w2, d_xn, gl.lntf, n, d_best,
+                                      g_dev.stream);
         if (rc == 2) {
             // table count without a specialised kernel: one pass per event
             for (int e = 0; e < n && ok; e++)
@@ -1996,7 +2013,7 @@ void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
                               d_sk, d_xn, gl.lntf, d_c, n, gl.ngrd,
                               g_dev.stream) == 0 &&
          d2h(t0v.data(), d_t0v, (size_t) n*sizeof(float));
-    dfree(d_o); dfree(d_w); dfree(d_xn); dfree(d_sk); dfree(d_c);
+    dfree(d_o); dfree(d_w); dfree(d_w2); dfree(d_xn); dfree(d_sk); dfree(d_c);
     dfree(d_t0v);
     if (!ok) { msg("locate_locateEvents: GPU error"); *ierr = 1; return; }
     for (int e = 0; e < n; e++)
@@ -2840,8 +2857,53 @@ struct Surrogate {
     unsigned short *d_dy = nullptr, *d_dz = nullptr;
     double *d_loss = nullptr;
     unsigned long long seed = 13;
+    // inversion work buffers (grown on demand): event arrays sized by
+    // events, the rest by rows R = events * starts
+    long invEvents = 0, invRows = 0;
+    float *d_io = nullptr, *d_iw = nullptr, *d_ixn = nullptr;
+    unsigned char *d_isk = nullptr;
+    unsigned long long *d_ibest = nullptr;
+    int *d_ifb = nullptr;
+    float *d_pos = nullptr, *d_mis = nullptr, *d_zp = nullptr,
+          *d_yall = nullptr;
+    unsigned short *d_ha = nullptr, *d_hb = nullptr;
 };
 Surrogate gs;
+
+void free_inv_work()
+{
+    dfree(gs.d_io); dfree(gs.d_iw); dfree(gs.d_ixn); dfree(gs.d_isk);
+    dfree(gs.d_ibest); dfree(gs.d_ifb);
+    dfree(gs.d_pos); dfree(gs.d_mis); dfree(gs.d_zp);
+    dfree(gs.d_yall); dfree(gs.d_ha); dfree(gs.d_hb);
+    gs.invEvents = gs.invRows = 0;
+}
+
+// Inversion buffers for n events (lntf observations each) and R rows.
+bool inv_work(long n, long lntf, long R)
+{
+    const int H = gs.H, T = gs.ntf;
+    if (n*lntf > gs.invEvents || R > gs.invRows) {
+        const long ne = std::max(n*lntf, gs.invEvents);
+        const long nr = std::max(R, gs.invRows);
+        const long nev = std::max(n, ne/std::max(lntf, 1L));
+        free_inv_work();
+        if (!(dalloc(&gs.d_io, (size_t) ne) && dalloc(&gs.d_iw, (size_t) ne) &&
+              dalloc(&gs.d_isk, (size_t) ne) && dalloc(&gs.d_ixn, (size_t) nev) &&
+              dalloc(&gs.d_ibest, (size_t) nev) && dalloc(&gs.d_ifb, (size_t) nev) &&
+              dalloc(&gs.d_pos, 3*(size_t) nr) && dalloc(&gs.d_mis, (size_t) nr) &&
+              dalloc(&gs.d_zp, (size_t) nr*H) &&
+              dalloc(&gs.d_yall, 4*(size_t) nr*T) &&
+              dalloc(&gs.d_ha, 4*(size_t) nr*H) &&
+              dalloc(&gs.d_hb, 4*(size_t) nr*H))) {
+            free_inv_work();
+            return false;
+        }
+        gs.invEvents = ne;
+        gs.invRows = nr;
+    }
+    return true;
+}
 
 void free_surrogate()
 {
@@ -2849,6 +2911,7 @@ void free_surrogate()
     dfree(gs.d_pb); dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y);
     dfree(gs.d_dh); dfree(gs.d_dy); dfree(gs.d_dz); dfree(gs.d_loss);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
+    free_inv_work();
     gs = Surrogate();
 }
 
@@ -3135,20 +3198,19 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
         memcpy(&sk[(size_t) e*gl.lntf], &gl.skip[k], (size_t) gl.lntf);
         xn[(size_t) e] = weight_norm(&gl.wt[k], gl.lntf);
     }
-    float *d_o = nullptr, *d_w = nullptr, *d_xn = nullptr, *d_pos = nullptr,
-          *d_mis = nullptr, *d_zall = nullptr, *d_zp = nullptr, *d_yall = nullptr;
-    unsigned short *d_ha = nullptr, *d_hb = nullptr;
-    unsigned char *d_sk = nullptr;
-    unsigned long long *d_best = nullptr;
-    int *d_fb = nullptr;
-    bool ok = dalloc(&d_o, o.size()) && dalloc(&d_w, w.size()) &&
-              dalloc(&d_xn, xn.size()) && dalloc(&d_sk, sk.size()) &&
-              dalloc(&d_pos, 3*(size_t) R) && dalloc(&d_mis, (size_t) R) &&
-              dalloc(&d_zall, 4*(size_t) R*H) && dalloc(&d_zp, (size_t) R*H) &&
-              dalloc(&d_yall, 4*(size_t) R*T) &&
-              dalloc(&d_ha, 4*(size_t) R*H) && dalloc(&d_hb, 4*(size_t) R*H) &&
-              dalloc(&d_best, (size_t) n) && dalloc(&d_fb, (size_t) n) &&
-              h2d(d_o, o.data(), o.size()*sizeof(float)) &&
+    if (!inv_work(n, gl.lntf, R)) {
+        msg("%s: out of GPU memory", who);
+        *ierr = 1;
+        return;
+    }
+    float *d_o = gs.d_io, *d_w = gs.d_iw, *d_xn = gs.d_ixn, *d_pos = gs.d_pos,
+          *d_mis = gs.d_mis, *d_zp = gs.d_zp,
+          *d_yall = gs.d_yall;
+    unsigned short *d_ha = gs.d_ha, *d_hb = gs.d_hb;
+    unsigned char *d_sk = gs.d_isk;
+    unsigned long long *d_best = gs.d_ibest;
+    int *d_fb = gs.d_ifb;
+    bool ok = h2d(d_o, o.data(), o.size()*sizeof(float)) &&
               h2d(d_w, w.data(), w.size()*sizeof(float)) &&
               h2d(d_xn, xn.data(), xn.size()*sizeof(float)) &&
               h2d(d_sk, sk.data(), sk.size());
@@ -3159,10 +3221,9 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
                                 d_zp, d_ha, H, s) == 0;
         unsigned short *hin = d_ha, *hout = d_hb;
         for (int l = 0; l < 3 && ok; l++) {
-            ok = fteik_k_gemm_bf16(hin, H, 1, gs.d_pb + gs.oW[l], 1, H, d_zall,
-                                   H, (int) (4*R), H, H, nullptr, 0, s) == 0 &&
-                 fteik_k_inv_act(d_zall, R, H, gs.d_p + gs.ob[l], d_zp, hout,
-                                 s) == 0;
+            // fused GEMM + activation (the GEMM then k_inv_act, bitwise)
+            ok = fteik_k_inv_hidden(hin, R, gs.d_pb + gs.oW[l],
+                                    gs.d_p + gs.ob[l], hout, H, s) == 0;
             std::swap(hin, hout);
         }
         ok = ok && fteik_k_gemm_bf16(hin, H, 1, gs.d_pb + gs.oW5, 1, H, d_yall,
@@ -3178,8 +3239,6 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
     std::vector<int> fb((size_t) n);
     ok = ok && d2h(best.data(), d_best, best.size()*sizeof(unsigned long long))
             && d2h(fb.data(), d_fb, fb.size()*sizeof(int));
-    dfree(d_pos); dfree(d_mis); dfree(d_zall); dfree(d_zp); dfree(d_yall);
-    dfree(d_ha); dfree(d_hb);
     // full grid search for the flagged events
     std::vector<int> redo;
     for (int e = 0; e < n && ok; e++) if (fb[(size_t) e]) redo.push_back(e);
@@ -3197,21 +3256,26 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
             memcpy(&sk2[(size_t) q*gl.lntf], &sk[e*gl.lntf], (size_t) gl.lntf);
             xn2[(size_t) q] = xn[e];
         }
-        float *d_o2 = nullptr, *d_w2 = nullptr, *d_xn2 = nullptr;
+        std::vector<float> h2((size_t) m*gl.lntf);
+        mask_skipped(o2.data(), w2.data(), h2.data(), sk2.data(), o2.size());
+        float *d_o2 = nullptr, *d_w2 = nullptr, *d_h2 = nullptr,
+              *d_xn2 = nullptr;
         unsigned char *d_sk2 = nullptr;
         unsigned long long *d_b2 = nullptr;
         ok = dalloc(&d_o2, o2.size()) && dalloc(&d_w2, w2.size()) &&
+             dalloc(&d_h2, h2.size()) &&
              dalloc(&d_xn2, xn2.size()) && dalloc(&d_sk2, sk2.size()) &&
              dalloc(&d_b2, b2.size()) &&
              h2d(d_o2, o2.data(), o2.size()*sizeof(float)) &&
              h2d(d_w2, w2.data(), w2.size()*sizeof(float)) &&
+             h2d(d_h2, h2.data(), h2.size()*sizeof(float)) &&
              h2d(d_xn2, xn2.data(), xn2.size()*sizeof(float)) &&
              h2d(d_sk2, sk2.data(), sk2.size()) &&
              h2d(d_b2, b2.data(), b2.size()*sizeof(unsigned long long));
         if (ok) {
             int rc = fteik_k_locate_batch(gl.d_tab, gl.ntf, gl.ngrd, gl.ldgrd,
-                                          d_o2, d_w2, d_sk2, d_xn2, gl.lntf, m,
-                                          d_b2, nullptr, s);
+                                          d_o2, d_w2, d_h2, d_xn2, gl.lntf, m,
+                                          d_b2, s);
             if (rc == 2)
                 for (int q = 0; q < m && ok; q++)
                     ok = fteik_k_locate(gl.d_tab, gl.ntf, gl.ngrd, gl.ldgrd,
@@ -3225,7 +3289,8 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
         }
         ok = ok && d2h(b2.data(), d_b2, b2.size()*sizeof(unsigned long long));
         for (int q = 0; q < m && ok; q++) best[(size_t) redo[(size_t) q]] = b2[(size_t) q];
-        dfree(d_o2); dfree(d_w2); dfree(d_xn2); dfree(d_sk2); dfree(d_b2);
+        dfree(d_o2); dfree(d_w2); dfree(d_h2); dfree(d_xn2); dfree(d_sk2);
+        dfree(d_b2);
     }
     // origin times at the chosen cells (locate.F90:684-688)
     std::vector<int> cells((size_t) n);
@@ -3238,16 +3303,15 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
         optIndx[e] = cells[(size_t) e] + 1;
         objOpt[e] = (double) obj;
     }
+    // the fallback flags and misfits are consumed: their buffers take the
+    // chosen cells and their origin times
     std::vector<float> t0v((size_t) n);
-    int *d_c = nullptr;
-    float *d_t0v = nullptr;
-    ok = ok && dalloc(&d_c, (size_t) n) && dalloc(&d_t0v, (size_t) n) &&
-         h2d(d_c, cells.data(), (size_t) n*sizeof(int)) &&
+    int *d_c = d_fb;
+    float *d_t0v = d_mis;
+    ok = ok && h2d(d_c, cells.data(), (size_t) n*sizeof(int)) &&
          fteik_k_locate_t0_at(d_t0v, gl.d_tab, gl.ntf, gl.ldgrd, d_o, d_w,
                               d_sk, d_xn, gl.lntf, d_c, n, gl.ngrd, s) == 0 &&
          d2h(t0v.data(), d_t0v, (size_t) n*sizeof(float));
-    dfree(d_c); dfree(d_t0v); dfree(d_o); dfree(d_w); dfree(d_xn); dfree(d_sk);
-    dfree(d_best); dfree(d_fb);
     if (!ok) { msg("%s: GPU error", who); *ierr = 1; return; }
     for (int e = 0; e < n; e++)
         t0Opt[e] = (double) t0v[(size_t) e] + gl.tepoch[(size_t) (evnmbr[e] - 1)];
@@ -3279,6 +3343,42 @@ void fteik_amd_gemmBf16(int M, int N, int K, const float *A, const float *B,
                                 0, g_dev.stream) == 0 &&
               d2h(C, dC, c*sizeof(float));
     dfree(dA); dfree(dB); dfree(dC); dfree(hA); dfree(hB);
+    if (!ok) *ierr = 1;
+}
+
+// Diagnostic: one hidden layer of the surrogate's inversion over 4 blocks
+// of R rows (primal, three tangents), H = 256.  hin (4R x 256), W (256 x
+// 256, row n holds output n's weights) and b (256) are rounded to BF16 /
+// kept FP32 as the network keeps them; fused receives the fused kernel's
+// BF16 output bits, unfused those of the MFMA GEMM followed by the
+// separate activation kernel.
+void fteik_amd_invHiddenBf16(int R, const float *hin, const float *W,
+                             const float *b, unsigned short *fused,
+                             unsigned short *unfused, int *ierr)
+{
+    *ierr = 0;
+    const int H = 256;
+    if (R < 1 || !device_ok("fteik_amd_invHiddenBf16")) { *ierr = 1; return; }
+    const size_t nh = 4*(size_t) R*H, nw = (size_t) H*H;
+    float *dh = nullptr, *dw = nullptr, *db = nullptr, *dz = nullptr,
+          *dzp = nullptr;
+    unsigned short *bh = nullptr, *bw = nullptr, *o1 = nullptr, *o2 = nullptr;
+    bool ok = dalloc(&dh, nh) && dalloc(&dw, nw) && dalloc(&db, (size_t) H) &&
+              dalloc(&dz, nh) && dalloc(&dzp, (size_t) R*H) &&
+              dalloc(&bh, nh) && dalloc(&bw, nw) && dalloc(&o1, nh) &&
+              dalloc(&o2, nh) &&
+              h2d(dh, hin, nh*sizeof(float)) && h2d(dw, W, nw*sizeof(float)) &&
+              h2d(db, b, H*sizeof(float)) &&
+              fteik_k_f2bf(dh, bh, (long) nh, g_dev.stream) == 0 &&
+              fteik_k_f2bf(dw, bw, (long) nw, g_dev.stream) == 0 &&
+              fteik_k_inv_hidden(bh, R, bw, db, o1, H, g_dev.stream) == 0 &&
+              fteik_k_gemm_bf16(bh, H, 1, bw, 1, H, dz, H, 4*R, H, H, nullptr,
+                                0, g_dev.stream) == 0 &&
+              fteik_k_inv_act(dz, R, H, db, dzp, o2, g_dev.stream) == 0 &&
+              d2h(fused, o1, nh*sizeof(unsigned short)) &&
+              d2h(unfused, o2, nh*sizeof(unsigned short));
+    dfree(dh); dfree(dw); dfree(db); dfree(dz); dfree(dzp); dfree(bh);
+    dfree(bw); dfree(o1); dfree(o2);
     if (!ok) *ierr = 1;
 }
 

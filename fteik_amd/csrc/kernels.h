@@ -123,10 +123,9 @@ int fteik_k_locate(const float* tables, int ntf, int ngrd, long ldgrd,
                    int accumulate, unsigned long long* best,
                    hipStream_t stream);
 int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
-                         const float* obs, const float* wt,
-                         const unsigned char* skip, const float* xnorm,
-                         int lntf, int nev, unsigned long long* best,
-                         float* t0best, hipStream_t stream);
+                         const float* obs, const float* wt, const float* wt2,
+                         const float* xnorm, int lntf, int nev,
+                         unsigned long long* best, hipStream_t s);
 
 // stencil_eval.hip (diagnostics: device stencils on given node inputs)
 int fteik_k_eval3d(const double* in, const int* node, const void* cst,
@@ -178,6 +177,8 @@ int fteik_k_sur_samples(const float* tab, long ldgrd, const void* geo,
                         hipStream_t stream);
 int fteik_k_sur_geo_size(void);
 int fteik_k_inv_start(float* pos, long R, int starts, hipStream_t stream);
+int fteik_k_inv_hidden(const void* hin, long R, const void* W,
+                       const float* b, void* hout, int H, hipStream_t s);
 int fteik_k_inv_layer1(const float* pos, long R, const float* W1,
                        const float* b1, float* zp, void* h, int H,
                        hipStream_t stream);

@@ -111,16 +111,21 @@ void k_locate_one(const float* __restrict__ tab, int ntf, int ngrd, long ldgrd,
 // removal of the grid search).  A thread holds CPT cells' tables in
 // registers and loops over the events; event data are uniform (scalar
 // loads).  best[e] receives the per-event key.
+//
+// Skipped picks arrive with obs = wt = wt2 = 0: their terms are +-0, and
+// adding a zero leaves an FP32 sum unchanged, so the sums equal the
+// reference's sums over the picks it keeps without a branch per pick.
+// wt2 = sqrt(1/2) * wt, the product the reference forms per pick.  Each
+// wave posts its minimum key with an atomic (no workgroup barrier per
+// event).
 template <int NTF, int CPT>
 __global__ __launch_bounds__(TL)
 void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
                     const float* __restrict__ obs, const float* __restrict__ wt,
-                    const unsigned char* __restrict__ skip,
+                    const float* __restrict__ wt2,
                     const float* __restrict__ xnorm, int lntf, int nev,
                     unsigned long long* __restrict__ best)
 {
-    __shared__ unsigned long long red[TL/64];
-    const float sqrtHalf = 0.70710677f;
     const long base = ((long) blockIdx.x*TL*CPT) + threadIdx.x;
     float T[CPT][NTF];
     #pragma unroll
@@ -133,14 +138,13 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
     for (int e = 0; e < nev; e++) {
         const float* o = obs + (long) e*lntf;
         const float* w = wt + (long) e*lntf;
-        const unsigned char* sk = skip + (long) e*lntf;
+        const float* w2 = wt2 + (long) e*lntf;
         const float xn = xnorm[e];
         float t0[CPT], ob[CPT];
         #pragma unroll
         for (int q = 0; q < CPT; q++) { t0[q] = 0.0f; ob[q] = 0.0f; }
         #pragma unroll
         for (int i = 0; i < NTF; i++) {
-            if (sk[i]) continue;
             const float oi = o[i], wi = w[i];
             #pragma unroll
             for (int q = 0; q < CPT; q++) {
@@ -150,8 +154,7 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
         }
         #pragma unroll
         for (int i = 0; i < NTF; i++) {
-            if (sk[i]) continue;
-            const float oi = o[i], wi = sqrtHalf*w[i];
+            const float oi = o[i], wi = w2[i];
             #pragma unroll
             for (int q = 0; q < CPT; q++) {
                 float est = T[q][i] + t0[q];
@@ -169,14 +172,7 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
             }
         }
         key = wave_min(key);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long k = red[0];
-            for (int i = 1; i < TL/64; i++) k = (red[i] < k) ? red[i] : k;
-            atomicMin(best + e, k);
-        }
-        __syncthreads();
+        if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(best + e, key);
     }
 }
 
@@ -231,16 +227,14 @@ int fteik_k_locate(const float* tables, int ntf, int ngrd, long ldgrd,
 }
 
 int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
-                         const float* obs, const float* wt,
-                         const unsigned char* skip, const float* xnorm,
-                         int lntf, int nev, unsigned long long* best,
-                         float* t0best, hipStream_t s)
+                         const float* obs, const float* wt, const float* wt2,
+                         const float* xnorm, int lntf, int nev,
+                         unsigned long long* best, hipStream_t s)
 {
-    (void) t0best;
     constexpr int CPT = 2;
     const int nblk = (int) ((ngrd + (long) TL*CPT - 1)/((long) TL*CPT));
 #define LB(N) case N: hipLaunchKernelGGL((k_locate_batch<N, CPT>), dim3(nblk), \
-        dim3(TL), 0, s, tables, ngrd, ldgrd, obs, wt, skip, xnorm, lntf, nev, \
+        dim3(TL), 0, s, tables, ngrd, ldgrd, obs, wt, wt2, xnorm, lntf, nev, \
         best); break;
     switch (ntf) {
         LB(1) LB(2) LB(3) LB(4) LB(5) LB(6) LB(7) LB(8) LB(10) LB(12) LB(16)

@@ -330,6 +330,69 @@ __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
     }
 }
 
+// Hidden layer of the inversion network, fused: 4 row blocks (primal and
+// three tangents) of R rows, H = 256.  z = h_in W^T on the MFMA units with
+// FP32 accumulation in the same k order as k_gemm_bf16 (so z is bitwise
+// the GEMM's), then h = silu(z + b) for the primal block and
+// h_d = silu'(z_0 + b) z_d for the tangents, rounded to BF16: what
+// k_gemm_bf16 followed by k_inv_act computes, without the FP32 round trip
+// through memory.  One workgroup per 32 rows; wave d computes block d with
+// its A rows held in registers over the whole K, the weights stream from
+// L2 as 16-byte vectors, and the primal wave hands silu' to the tangent
+// waves through LDS one 32-column tile at a time.
+__global__ __launch_bounds__(256)
+void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
+                  const bf16_t* __restrict__ W, const float* __restrict__ b,
+                  bf16_t* __restrict__ hout)
+{
+    constexpr int H = 256, KS = H/16;
+    __shared__ float ds[2][32][33];
+    const int lane = threadIdx.x & 63, d = threadIdx.x >> 6;
+    const long s0 = (long) blockIdx.x*32;
+    const long arow = s0 + (lane & 31);
+    const bool aok = arow < R;
+    const bf16_t* ap = hin + ((long) d*R + (aok ? arow : 0))*H + 8*(lane >> 5);
+    bf16x8 a[KS];
+    #pragma unroll
+    for (int kk = 0; kk < KS; kk++) {
+        bf16x8 v = {};
+        if (aok) v = *(const bf16x8*) (ap + 16*kk);
+        a[kk] = v;
+    }
+    for (int ct = 0; ct < H/32; ct++) {
+        const int n = 32*ct + (lane & 31);
+        const bf16_t* wp = W + (long) n*H + 8*(lane >> 5);
+        f32x16 acc = {};
+        #pragma unroll
+        for (int kk = 0; kk < KS; kk++) {
+            const bf16x8 bv = *(const bf16x8*) (wp + 16*kk);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk], bv, acc,
+                                                          0, 0, 0);
+        }
+        const float bn = b[n];
+        float (*dsb)[33] = ds[ct & 1];
+        if (d == 0) {
+            #pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+                const float z = acc[r] + bn;
+                dsb[m][lane & 31] = dsilu(z);
+                if (s0 + m < R) hout[(s0 + m)*H + n] = f2bf(silu(z));
+            }
+        }
+        __syncthreads();
+        if (d > 0) {
+            #pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+                if (s0 + m < R)
+                    hout[((long) d*R + s0 + m)*H + n] =
+                        f2bf(dsb[m][lane & 31]*acc[r]);
+            }
+        }
+    }
+}
+
 // After a hidden GEMM over all 4R rows (no bias): primal h = silu(z + b),
 // tangents h_d = silu'(z + b) * z_d.
 __global__ void k_inv_act(const float* __restrict__ zall, long R, int H,
@@ -619,6 +682,16 @@ int fteik_k_inv_start(float* pos, long R, int starts, hipStream_t s)
     hipLaunchKernelGGL(k_inv_start, dim3(nblocks(R)), dim3(TPB), 0, s, pos, R,
                        starts);
     return ok();
+}
+
+int fteik_k_inv_hidden(const void* hin, long R, const void* W,
+                       const float* b, void* hout, int H, hipStream_t s)
+{
+    if (H != 256 || R < 1) return 2;
+    hipLaunchKernelGGL(k_inv_hidden, dim3((unsigned) ((R + 31)/32)),
+                       dim3(256), 0, s, (const bf16_t*) hin, R,
+                       (const bf16_t*) W, b, (bf16_t*) hout);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int fteik_k_inv_layer1(const float* pos, long R, const float* W1,

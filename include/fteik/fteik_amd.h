@@ -286,6 +286,13 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
 /* diagnostic: C = A B (row-major FP32 in/out) through the BF16 MFMA GEMM */
 void fteik_amd_gemmBf16(int M, int N, int K, const float *A, const float *B,
                         float *C, int *ierr);
+/* EXTENSION (diagnostic; no counterpart in the reference): one hidden layer
+ * of the surrogate's inversion (4 blocks of R rows: primal and three
+ * tangents, 256 wide) by the fused MFMA kernel and by the GEMM followed by
+ * the activation kernel; BF16 output bits of both (4R x 256 each). */
+void fteik_amd_invHiddenBf16(int R, const float *hin, const float *W,
+                             const float *b, unsigned short *fused,
+                             unsigned short *unfused, int *ierr);
 
 /* =========================== Model helpers =========================== */
 /* model.F90:771 */

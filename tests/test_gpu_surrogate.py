@@ -37,6 +37,35 @@ def test_gemm_bf16_matches_numpy(M, N, K):
     assert np.all(np.abs(C - ref) <= tol), np.max(np.abs(C - ref) / tol)
 
 
+@pytest.mark.parametrize("R", [32, 45, 1000])
+def test_fused_hidden_layer(R):
+    """The fused inversion layer equals the GEMM + activation kernels bit
+    for bit, and the float64 statement of the layer on the BF16 inputs to
+    BF16 rounding (silu'(z_0) z_d for the tangent blocks)."""
+    rng = np.random.default_rng(R)
+    H = 256
+    hin = rng.standard_normal((4 * R, H)).astype(np.float32)
+    W = (rng.standard_normal((H, H)) / 16).astype(np.float32)
+    b = (0.1 * rng.standard_normal(H)).astype(np.float32)
+    fused = np.zeros((4 * R, H), dtype=np.uint16)
+    unfused = np.zeros((4 * R, H), dtype=np.uint16)
+    ierr = c_int(1)
+    up = ctypes.POINTER(ctypes.c_ushort)
+    lib().fteik_amd_invHiddenBf16(R, _f(hin), _f(W), _f(b),
+                                  fused.ctypes.data_as(up),
+                                  unfused.ctypes.data_as(up), byref(ierr))
+    assert ierr.value == 0
+    np.testing.assert_array_equal(fused, unfused)
+    got = (fused.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    z = mlp.bf16(hin).astype(np.float64) @ mlp.bf16(W).astype(np.float64).T
+    z0 = z[:R] + b
+    sig = 1.0 / (1.0 + np.exp(-z0))
+    want = np.concatenate([z0 * sig] + [sig * (1 + z0 * (1 - sig)) * z[d * R:(d + 1) * R]
+                                        for d in (1, 2, 3)])
+    tol = 1e-2 * np.abs(want) + 1e-3 * np.abs(want).max()
+    assert np.all(np.abs(got - want) <= tol), np.max(np.abs(got - want) / tol)
+
+
 def _init_surrogate(ntf, seed=3):
     L = lib()
     ierr = c_int(1)
