@@ -15,9 +15,11 @@ each timed over K steps after W warm-up steps.
 
 Output: one JSON line.  value = node updates per second of the FP64 solve
 (updates as the reference counts them: initialization masks plus
-nsweep x 8 x (n-1)^3), in Gupdates/s.  The mixed path runs nsweep FP32
-iterations and FP64 iterations until one changes nothing.  The mixed path's rate on the same
-update count, its speed-up and its error against the FP64 field are
+nsweep x 8 x (n-1)^3), in Gupdates/s, for the dense sweeps (every update
+performed).  The same solve with tile skipping and with chained sweeps is
+reported as time to the same field.  The mixed path runs nsweep FP32
+iterations and FP64 iterations until one changes nothing; its rate on the
+same update count, its speed-up and its error against the FP64 field are
 reported beside it.  roofline: the FP64 Gauss-Seidel sweep kernel, timed
 with hipEvents on the library's stream; cpu_baseline: the oracle's OpenMP
 level-synchronous FP64 solve of the same workload on the host cores.
@@ -107,7 +109,7 @@ def run_batch(L, models, h, nsweep, tile):
 
 
 def batch_section(L, ntiles):
-    """Config 3: 64 independent grids of uneven size swept together; the
+    """Batched solve: 64 independent grids of uneven size swept together; the
     tile shape is picked by timing every shape on the batch (autotuner)."""
     h = 50.0
     models = batch_models()
@@ -133,10 +135,11 @@ def batch_section(L, ntiles):
 
 
 def locator_workload():
-    """Config 4: 161 x 161 x 81 (nx = ny = 161, nz = 81), h = 250 m,
-    v = 4000 + 0.4 z; 48 stations at z = 0, x, y ~ U[0, 158 h) (seed 9);
-    10,000 events at uniform random nodes (seed 10), origin times U(0, 100) s
-    (seed 11), picks = T + origin + N(0, 0.05^2) (seed 12), weights 1."""
+    """Grid-search workload: 161 x 161 x 81 (nx = ny = 161, nz = 81),
+    h = 250 m, v = 4000 + 0.4 z; 48 stations at z = 0, x, y ~ U[0, 158 h)
+    (seed 9); 10,000 events at uniform random nodes (seed 10), origin
+    times U(0, 100) s (seed 11), picks = T + origin + N(0, 0.05^2)
+    (seed 12), weights 1."""
     nz, nx, ny, h = 81, 161, 161, 250.0
     zc = (np.arange(nz - 1) + 0.5) * h
     vel = np.ascontiguousarray(np.tile(4000.0 + 0.4 * zc,
@@ -230,9 +233,10 @@ def locator_section(L, args, nev=10000, cpu_events=16):
 
 
 def surrogate_section(L, stations, grid, nev, args, full_opt, full_s):
-    """Config 5: MLP 3 -> 256^4 -> 48 trained on 2M (cell, times) pairs from
-    the locator's tables (seed 13), then Gauss-Newton proposals scored with
-    the exact tables; agreement and events/s against the full search."""
+    """Learned locator: MLP 3 -> 256^4 -> 48 trained on 2M (cell, times)
+    pairs from the locator's tables (seed 13), then Gauss-Newton proposals
+    scored with the exact tables; agreement and events/s against the full
+    search."""
     nz, nx, ny, h = grid
     ierr = c_int(1)
     dp = ctypes.POINTER(c_double)

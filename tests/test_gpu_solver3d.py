@@ -72,13 +72,13 @@ def test_solve_matches_oracle(case, schedule):
     assert ref.max() < orc.HUGE
 
 
-def test_config1_linear_gradient_101():
+def test_linear_gradient_101():
     """The 101^3 workload: h = 10 m, v(z) = 3000 + 0.5 z, source at the
     centre, 20 seeded surface receivers, 2 iterations.  The field equals
     the oracle's bit for bit, receivers too, and stays within first-order
     accuracy of the closed form (analytic.F90:456-528)."""
     n, h = 101, 10.0
-    _announce("config1", n)
+    _announce("linear gradient", n)
     zc = (np.arange(n - 1) + 0.5) * h
     vel = np.tile(3000.0 + 0.5 * zc, (n - 1) * (n - 1))
     zs = xs = ys = 50 * h
