@@ -59,6 +59,79 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
     pencil_body<R, INIT, BX, BY, HB>(a, m.tile);
 }
 
+// Iteration sweep of a batch with mailbox tiles (modes 1, 2): workgroup w
+// sweeps map entry (grid, tile); entries are ordered by tile diagonal,
+// then grid, then x, so every workgroup waits only on lower-numbered ones
+// and in-order dispatch keeps the lowest unfinished one resident (the
+// launch is larger than the GPU).  args[g]: grid g's sweep arguments for
+// this launch; each grid has its own mailbox.
+template <typename R, int BX, int BY, int HB, int MODE>
+__global__ __launch_bounds__(BX*BY + 64*MODE)
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
+void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
+                     const int* __restrict__ map)
+{
+    // read through the constant address space: the loads are invariant, so
+    // the register allocator may reload a field where it is used (as it
+    // does kernel arguments) instead of keeping it live
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(4))) SweepArgs<R> CArgs;
+    const SweepArgs<R> a = ((CArgs*) args)[map[2*blockIdx.x]];
+#else   // the host pass only parses the kernel
+    const SweepArgs<R> a = args[map[2*blockIdx.x]];
+#endif
+    pencil_mb_body<R, false, BX, BY, HB, MODE, false>(a, dyn_of(a),
+                                                      map[2*blockIdx.x + 1]);
+}
+
+// Sweep arguments of every grid for one mailbox iteration sweep.
+template <typename R>
+__global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
+                                     int ngrids, int sweep, int bx, int by,
+                                     unsigned long long serial,
+                                     int* __restrict__ err, int fp32,
+                                     int track,
+                                     SweepArgs<R>* __restrict__ args)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= ngrids) return;
+    const BatchGrid g = grids[i];
+    SweepArgs<R> a;
+    a.tt = (R*) (fp32 ? g.tt32 : g.tt);
+    a.slow = (const R*) (fp32 ? g.slow32 : g.slow);
+    a.cst = (const Consts<R>*) (fp32 ? g.cst32 : g.cst);
+    a.flags = nullptr;
+    a.err = err;
+    a.serial = serial;
+    a.nz = g.nz; a.nx = g.nx; a.ny = g.ny;
+    sweep_signs(sweep, a.sz, a.sx, a.sy);
+    int b[6];
+    flipped_box(sweep, false, g.nz, g.nx, g.ny, 1, 1, 1, b);
+    a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
+    a.fy0 = b[4]; a.fy1 = b[5];
+    a.nbx = (a.fx1 - a.fx0 + bx)/bx;
+    a.nby = (a.fy1 - a.fy0 + by)/by;
+    a.maxdelta = track ? g.maxdelta : nullptr;
+    a.mb = g.mb;
+    a.colmod = nullptr; a.plan = nullptr; a.prog = nullptr;
+    a.pserial = 0; a.par = 0; a.psz = a.psx = a.psy = 0;
+    args[i] = a;
+}
+
+template <typename R>
+static const void* batch_mb_fn(int tile)
+{
+#define BM(n) case n: return (const void*) batch_mb_kernel<R, kTiles[n].bx, \
+        kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
+    switch (tile) {
+        BM(7) BM(8) BM(9) BM(10) BM(11) BM(12) BM(13)
+        default: return nullptr;
+    }
+#undef BM
+}
+static_assert(kTiles[7].mode == 1 && kTiles[13].mode == 2 && kNumTiles == 14,
+              "batch_mb_fn lists the mailbox tiles");
+
 template <typename R, bool INIT>
 static const void* batch_fn_t(int tile)
 {
@@ -133,6 +206,76 @@ extern "C" int fteik_k_batch_sweep(const BatchGrid* grids, const int* map,
                     (void*) &serial, (void*) &flags, (void*) &err,
                     (void*) &fp32, (void*) &track};
     hipError_t e = hipLaunchKernel(fn, dim3(nwg), dim3(ts.bx*ts.by), args, 0,
+                                   stream);
+    return e == hipSuccess ? 0 : 1;
+}
+
+// Workgroup order of a mailbox iteration sweep of a batch for tile shape
+// `tile` (iteration box: flipped columns 1 .. n-1): entries (grid, tile) by
+// diagonal, then grid, then x; 2 ints each.  Returns the entry count (with
+// map == nullptr nothing is written), or -1 if map_cap is too small.
+// Host-side bookkeeping only.
+extern "C" int fteik_k_batch_mb_map(int tile, int ngrids, const int* dims,
+                                    int* map, int map_cap)
+{
+    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0) return -1;
+    const TileShape ts = kTiles[tile];
+    int total = 0, maxdiag = 0;
+    for (int g = 0; g < ngrids; g++) {
+        const int nbx = (dims[3*g + 1] - 1 + ts.bx - 1)/ts.bx;
+        const int nby = (dims[3*g + 2] - 1 + ts.by - 1)/ts.by;
+        total += nbx*nby;
+        maxdiag = (nbx + nby - 2 > maxdiag) ? nbx + nby - 2 : maxdiag;
+    }
+    if (!map) return total;
+    if (total > map_cap) return -1;
+    int w = 0;
+    for (int d = 0; d <= maxdiag; d++)
+        for (int g = 0; g < ngrids; g++) {
+            const int nbx = (dims[3*g + 1] - 1 + ts.bx - 1)/ts.bx;
+            const int nby = (dims[3*g + 2] - 1 + ts.by - 1)/ts.by;
+            for (int bx = 0; bx < nbx; bx++) {
+                const int by = d - bx;
+                if (by < 0 || by >= nby) continue;
+                map[2*w] = g;
+                map[2*w + 1] = bx*nby + by;
+                w++;
+            }
+        }
+    return w;
+}
+
+extern "C" size_t fteik_k_batch_mb_args_bytes(int ngrids)
+{
+    return (size_t) ngrids*sizeof(SweepArgs<double>);
+}
+
+// One mailbox iteration sweep of a batch.  argbuf: device buffer of
+// fteik_k_batch_mb_args_bytes(ngrids) bytes; map: nwg entries from
+// fteik_k_batch_mb_map.
+extern "C" int fteik_k_batch_mb_sweep(const BatchGrid* grids, int ngrids,
+                                      const int* map, int nwg, int tile,
+                                      int sweep, unsigned long long serial,
+                                      int* err, int fp32, int track,
+                                      void* argbuf, hipStream_t stream)
+{
+    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0 || nwg < 1 ||
+        ngrids < 1 || sweep < 1 || sweep > 8 || !argbuf) return 2;
+    const TileShape ts = kTiles[tile];
+    const void* fn = fp32 ? batch_mb_fn<float>(tile) : batch_mb_fn<double>(tile);
+    if (fp32)
+        hipLaunchKernelGGL(batch_mb_args_kernel<float>, dim3((ngrids + 63)/64),
+                           dim3(64), 0, stream, grids, ngrids, sweep, ts.bx,
+                           ts.by, serial, err, fp32, track,
+                           (SweepArgs<float>*) argbuf);
+    else
+        hipLaunchKernelGGL(batch_mb_args_kernel<double>,
+                           dim3((ngrids + 63)/64), dim3(64), 0, stream, grids,
+                           ngrids, sweep, ts.bx, ts.by, serial, err, fp32,
+                           track, (SweepArgs<double>*) argbuf);
+    void* args[] = {&argbuf, (void*) &map};
+    hipError_t e = hipLaunchKernel(fn, dim3(nwg),
+                                   dim3(ts.bx*ts.by + 64*ts.mode), args, 0,
                                    stream);
     return e == hipSuccess ? 0 : 1;
 }

@@ -33,8 +33,11 @@ def _batch(grids, models, nsweep, eps, tile=None, precision=0):
         L.fteik_batch3d_setGrid(g + 1, nz, nx, ny, 0.0, 0.0, 0.0, h, h, h,
                                 _d(v), zc * h, xc * h, yc * h, byref(ierr))
         assert ierr.value == 0
-    if tile is not None:
-        L.fteik_batch3d_setTile(tile, byref(ierr))
+    # an int, or a sequence: a flag tile for the initialization sweeps
+    # followed by a mailbox tile for the iteration sweeps
+    for t in ([] if tile is None else
+              [tile] if isinstance(tile, int) else tile):
+        L.fteik_batch3d_setTile(t, byref(ierr))
         assert ierr.value == 0
     L.fteik_batch3d_setPrecision(precision, byref(ierr))
     L.fteik_batch3d_solve(nsweep, eps, byref(ierr))
@@ -50,7 +53,8 @@ def _field(L, g, n):
     return tt
 
 
-@pytest.mark.parametrize("tile", [0, 3, 6])
+@pytest.mark.parametrize("tile", [0, 3, 6, 7, 8, 9, 10, 11, 12, 13,
+                                  (3, 12), (0, 10)])
 def test_batch_matches_oracle(tile):
     print("case: batch tile", tile, flush=True)
     models = [hetero(nz, nx, ny, h, 40 + k)
@@ -65,6 +69,25 @@ def test_batch_matches_oracle(tile):
     L.fteik_batch3d_getLastSolveInfo(byref(ms), byref(corr), byref(upd),
                                      byref(ierr))
     assert ierr.value == 0 and upd.value > 0 and ms.value > 0
+    L.fteik_batch3d_free()
+
+
+@pytest.mark.parametrize("tile", [6, 13, 9])
+def test_large_batch_matches_oracle(tile):
+    """48 grids of 65^3 (four sources, each twelve times): more workgroups
+    per sweep than the GPU holds at once, so later grids' tiles wait for
+    earlier ones to leave; every grid equals the oracle bit for bit."""
+    print("case: large batch tile", tile, flush=True)
+    n, h = 65, 25.0
+    vel = hetero(n, n, n, h, 91)
+    sources = [(3.3, 60.2, 31.0), (32.0, 32.0, 32.0), (61.5, 2.5, 7.25),
+               (0.0, 44.4, 62.0)]
+    grids = [(n, n, n, h, sources[k % 4]) for k in range(48)]
+    L = _batch(grids, [vel] * 48, 2, 5.0, tile=tile)
+    refs = [oracle3d(vel, n, n, n, h, zc * h, xc * h, yc * h, 2, 5.0)[0]
+            for (zc, xc, yc) in sources]
+    for k in range(48):
+        np.testing.assert_array_equal(_field(L, k, n ** 3), refs[k % 4])
     L.fteik_batch3d_free()
 
 
@@ -138,7 +161,7 @@ def test_batch_error_paths():
     L.fteik_batch3d_free()
 
 
-@pytest.mark.parametrize("tile", [3, 6])
+@pytest.mark.parametrize("tile", [3, 6, 11, 13])
 def test_batch_mixed_precision_matches_fp64(tile):
     """Mixed mode (FP32 iterations, FP64 corrections to convergence) of a
     batch: every grid is within 1e-6
