@@ -164,12 +164,29 @@ void skip_plan_kernel(const unsigned long long* __restrict__ colmod,
         s_dirty[t] = 0;
     }
     __syncthreads();
-    const int lx = fx1 - fx0 + 2, ly = fy1 - fy0 + 2;
-    for (int c = threadIdx.x; c < lx*ly; c += blockDim.x) {
-        const int fk = fy0 - 1 + c / lx, fj = fx0 - 1 + c % lx;
-        const int t = (max(fj - fx0, 0)/bx)*nby + max(fk - fy0, 0)/by;
-        const int j = unflip(fj, nx, sx), k = unflip(fk, ny, sy);
-        if (colmod[(long) (k - 1)*nx + (j - 1)] > s_last[t]) s_dirty[t] = 1;
+    // every column of the box and the row before it; each thread has PU
+    // stamp loads in flight before it tests any of them
+    constexpr int PU = 8;
+    const int lx = fx1 - fx0 + 2, ly = fy1 - fy0 + 2, ncol = lx*ly;
+    for (int c0 = threadIdx.x; c0 < ncol; c0 += PU*blockDim.x) {
+        unsigned long long m[PU];
+        int tl[PU];
+        #pragma unroll
+        for (int u = 0; u < PU; u++) {
+            const int c = c0 + u*blockDim.x;
+            m[u] = 0;
+            tl[u] = 0;
+            if (c < ncol) {
+                const int fk = fy0 - 1 + c / lx, fj = fx0 - 1 + c % lx;
+                tl[u] = (max(fj - fx0, 0)/bx)*nby + max(fk - fy0, 0)/by;
+                const int j = unflip(fj, nx, sx), k = unflip(fk, ny, sy);
+                m[u] = colmod[(long) (k - 1)*nx + (j - 1)];
+            }
+        }
+        #pragma unroll
+        for (int u = 0; u < PU; u++)
+            if (c0 + u*blockDim.x < ncol && m[u] > s_last[tl[u]])
+                s_dirty[tl[u]] = 1;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < nbx; p += blockDim.x) {
