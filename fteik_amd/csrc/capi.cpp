@@ -620,11 +620,11 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     for (int t = 0; t < ntile; t++)
         g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
                                         fteik_k_pencil_fits(nx, ny, 1, t));
-    // default: the first available shape in order of measured speed.  When
-    // 16x8 tiles give every CU at least one workgroup (257^3: 512 tiles),
-    // 16x8 mailbox tiles are fastest; on smaller grids the 8x8 tiles keep
-    // more CUs busy.
-    static const int pref_large[] = {8, 11, 9, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
+    // default: the first available shape in order of measured speed (FP64
+    // sweep of a 257^3 grid: 8x8 mailbox tiles 1.10 ms, 16x8 1.14 ms, 16x8
+    // with a store wave 1.15 ms); on smaller grids the 8x8 tiles keep more
+    // CUs busy as well.
+    static const int pref_large[] = {9, 8, 11, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
     static const int pref_small[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
     const bool large = (long) (nx - 1)*(ny - 1) >= 128L*g_dev.ncu;
     int first = -1;

@@ -446,15 +446,21 @@ __device__ __forceinline__ void pencil_body(const SweepArgs<R>& a, const int wg)
 // reader can tell this sweep's value from an older one without any
 // ordering between separate stores: FP64 uses two words (low and high
 // halves), FP32 one.
+// FP64: both words go out in one 16-byte device-coherent (sc1) store and
+// come back in one such load: half the memory requests of two 8-byte
+// accesses.  Each word still carries the tag, so a reader that sees the two
+// halves from different stores rejects the pair.
+typedef unsigned long long MbPair __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void mb_put(unsigned long long* mb, long g,
                                        double v, unsigned tag)
 {
     const unsigned long long b = (unsigned long long) __double_as_longlong(v);
     const unsigned long long t = (unsigned long long) tag << 32;
-    __hip_atomic_store(mb + 2*g, t | (b & 0xffffffffull), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(mb + 2*g + 1, t | (b >> 32), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    MbPair w;
+    w.x = t | (b & 0xffffffffull);
+    w.y = t | (b >> 32);
+    asm volatile("global_store_dwordx4 %0, %1, off sc1"
+                 : : "v"(mb + 2*g), "v"(w) : "memory");
 }
 __device__ __forceinline__ void mb_put(unsigned long long* mb, long g,
                                        float v, unsigned tag)
@@ -466,11 +472,10 @@ __device__ __forceinline__ void mb_put(unsigned long long* mb, long g,
 __device__ __forceinline__ bool mb_get(unsigned long long* mb, long g,
                                        unsigned tag, double& v)
 {
-    const unsigned long long lo = __hip_atomic_load(mb + 2*g, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long hi = __hip_atomic_load(mb + 2*g + 1,
-                                                    __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+    MbPair w;
+    asm volatile("global_load_dwordx4 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=v"(w) : "v"(mb + 2*g) : "memory");
+    const unsigned long long lo = w.x, hi = w.y;
     if ((unsigned) (lo >> 32) != tag || (unsigned) (hi >> 32) != tag)
         return false;
     v = __longlong_as_double((long long) ((hi << 32) | (lo & 0xffffffffull)));
