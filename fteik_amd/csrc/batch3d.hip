@@ -59,13 +59,13 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
     pencil_body<R, INIT, BX, BY, HB>(a, m.tile);
 }
 
-// Iteration sweep of a batch with mailbox tiles (modes 1, 2): workgroup w
-// sweeps map entry (grid, tile); entries are ordered by tile diagonal,
-// then grid, then x, so every workgroup waits only on lower-numbered ones
-// and in-order dispatch keeps the lowest unfinished one resident (the
-// launch is larger than the GPU).  args[g]: grid g's sweep arguments for
-// this launch; each grid has its own mailbox.
-template <typename R, int BX, int BY, int HB, int MODE>
+// Sweep of a batch with mailbox tiles (modes 1, 2; INIT: initialization
+// sweep): workgroup w sweeps map entry (grid, tile); entries are ordered by
+// tile diagonal, then grid, then x, so every workgroup waits only on
+// lower-numbered ones and in-order dispatch keeps the lowest unfinished one
+// resident (the launch is larger than the GPU).  args[g]: grid g's sweep
+// arguments for this launch; each grid has its own mailbox.
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
 void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
@@ -80,14 +80,16 @@ void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
 #else   // the host pass only parses the kernel
     const SweepArgs<R> a = args[map[2*blockIdx.x]];
 #endif
-    pencil_mb_body<R, false, BX, BY, HB, MODE, false>(a, dyn_of(a),
-                                                      map[2*blockIdx.x + 1]);
+    pencil_mb_body<R, INIT, BX, BY, HB, MODE, false>(a, dyn_of(a),
+                                                     map[2*blockIdx.x + 1]);
 }
 
-// Sweep arguments of every grid for one mailbox iteration sweep.
+// Sweep arguments of every grid for one mailbox sweep (linit: an
+// initialization sweep, whose box depends on the grid's source).
 template <typename R>
 __global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
-                                     int ngrids, int sweep, int bx, int by,
+                                     int ngrids, int sweep, int linit,
+                                     int bx, int by,
                                      unsigned long long serial,
                                      int* __restrict__ err, int fp32,
                                      int track,
@@ -106,11 +108,11 @@ __global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
     a.nz = g.nz; a.nx = g.nx; a.ny = g.ny;
     sweep_signs(sweep, a.sz, a.sx, a.sy);
     int b[6];
-    flipped_box(sweep, false, g.nz, g.nx, g.ny, 1, 1, 1, b);
+    flipped_box(sweep, linit != 0, g.nz, g.nx, g.ny, g.zsi, g.xsi, g.ysi, b);
     a.fz0 = b[0]; a.fz1 = b[1]; a.fx0 = b[2]; a.fx1 = b[3];
     a.fy0 = b[4]; a.fy1 = b[5];
-    a.nbx = (a.fx1 - a.fx0 + bx)/bx;
-    a.nby = (a.fy1 - a.fy0 + by)/by;
+    a.nbx = max((a.fx1 - a.fx0 + bx)/bx, 0);
+    a.nby = max((a.fy1 - a.fy0 + by)/by, 0);
     a.maxdelta = track ? g.maxdelta : nullptr;
     a.mb = g.mb;
     a.colmod = nullptr; a.plan = nullptr; a.prog = nullptr;
@@ -118,11 +120,11 @@ __global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
     args[i] = a;
 }
 
-template <typename R>
+template <typename R, bool INIT>
 static const void* batch_mb_fn(int tile)
 {
-#define BM(n) case n: return (const void*) batch_mb_kernel<R, kTiles[n].bx, \
-        kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
+#define BM(n) case n: return (const void*) batch_mb_kernel<R, INIT, \
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
     switch (tile) {
         BM(7) BM(8) BM(9) BM(10) BM(11) BM(12) BM(13)
         default: return nullptr;
@@ -210,30 +212,49 @@ extern "C" int fteik_k_batch_sweep(const BatchGrid* grids, const int* map,
     return e == hipSuccess ? 0 : 1;
 }
 
-// Workgroup order of a mailbox iteration sweep of a batch for tile shape
-// `tile` (iteration box: flipped columns 1 .. n-1): entries (grid, tile) by
-// diagonal, then grid, then x; 2 ints each.  Returns the entry count (with
-// map == nullptr nothing is written), or -1 if map_cap is too small.
-// Host-side bookkeeping only.
+// Workgroup order of a mailbox sweep of a batch for tile shape `tile`:
+// entries (grid, tile) by diagonal, then grid, then x; 2 ints each.  sweep
+// 0: an iteration sweep (box: flipped columns 1 .. n-1, the same for every
+// direction); sweep 1..8: that initialization sweep, whose box runs from
+// each grid's source (srcs: zsi, xsi, ysi per grid, 1-based) and may be
+// empty.  Returns the entry count (with map == nullptr nothing is
+// written), or -1 if map_cap is too small.  Host-side bookkeeping only.
 extern "C" int fteik_k_batch_mb_map(int tile, int ngrids, const int* dims,
-                                    int* map, int map_cap)
+                                    const int* srcs, int sweep, int* map,
+                                    int map_cap)
 {
-    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0) return -1;
+    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0 ||
+        sweep < 0 || sweep > 8 || (sweep > 0 && !srcs)) return -1;
     const TileShape ts = kTiles[tile];
+    auto counts = [&](int g, int& nbx, int& nby) {
+        const int nz = dims[3*g], nx = dims[3*g + 1], ny = dims[3*g + 2];
+        if (sweep == 0) {
+            nbx = (nx - 1 + ts.bx - 1)/ts.bx;
+            nby = (ny - 1 + ts.by - 1)/ts.by;
+            return;
+        }
+        int b[6];
+        flipped_box(sweep, true, nz, nx, ny, srcs[3*g], srcs[3*g + 1],
+                    srcs[3*g + 2], b);
+        const bool empty = b[0] > b[1] || b[2] > b[3] || b[4] > b[5];
+        nbx = empty ? 0 : (b[3] - b[2] + ts.bx)/ts.bx;
+        nby = empty ? 0 : (b[5] - b[4] + ts.by)/ts.by;
+    };
     int total = 0, maxdiag = 0;
     for (int g = 0; g < ngrids; g++) {
-        const int nbx = (dims[3*g + 1] - 1 + ts.bx - 1)/ts.bx;
-        const int nby = (dims[3*g + 2] - 1 + ts.by - 1)/ts.by;
+        int nbx, nby;
+        counts(g, nbx, nby);
         total += nbx*nby;
-        maxdiag = (nbx + nby - 2 > maxdiag) ? nbx + nby - 2 : maxdiag;
+        if (nbx > 0 && nby > 0)
+            maxdiag = (nbx + nby - 2 > maxdiag) ? nbx + nby - 2 : maxdiag;
     }
     if (!map) return total;
     if (total > map_cap) return -1;
     int w = 0;
     for (int d = 0; d <= maxdiag; d++)
         for (int g = 0; g < ngrids; g++) {
-            const int nbx = (dims[3*g + 1] - 1 + ts.bx - 1)/ts.bx;
-            const int nby = (dims[3*g + 2] - 1 + ts.by - 1)/ts.by;
+            int nbx, nby;
+            counts(g, nbx, nby);
             for (int bx = 0; bx < nbx; bx++) {
                 const int by = d - bx;
                 if (by < 0 || by >= nby) continue;
@@ -255,24 +276,30 @@ extern "C" size_t fteik_k_batch_mb_args_bytes(int ngrids)
 // fteik_k_batch_mb_map.
 extern "C" int fteik_k_batch_mb_sweep(const BatchGrid* grids, int ngrids,
                                       const int* map, int nwg, int tile,
-                                      int sweep, unsigned long long serial,
+                                      int sweep, int linit,
+                                      unsigned long long serial,
                                       int* err, int fp32, int track,
                                       void* argbuf, hipStream_t stream)
 {
-    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0 || nwg < 1 ||
+    if (tile < 0 || tile >= kNumTiles || kTiles[tile].mode == 0 || nwg < 0 ||
         ngrids < 1 || sweep < 1 || sweep > 8 || !argbuf) return 2;
+    if (nwg == 0) return 0;                 // every grid's box is empty
     const TileShape ts = kTiles[tile];
-    const void* fn = fp32 ? batch_mb_fn<float>(tile) : batch_mb_fn<double>(tile);
+    const void* fn = linit
+        ? (fp32 ? batch_mb_fn<float, true>(tile)
+                : batch_mb_fn<double, true>(tile))
+        : (fp32 ? batch_mb_fn<float, false>(tile)
+                : batch_mb_fn<double, false>(tile));
     if (fp32)
         hipLaunchKernelGGL(batch_mb_args_kernel<float>, dim3((ngrids + 63)/64),
-                           dim3(64), 0, stream, grids, ngrids, sweep, ts.bx,
-                           ts.by, serial, err, fp32, track,
+                           dim3(64), 0, stream, grids, ngrids, sweep, linit,
+                           ts.bx, ts.by, serial, err, fp32, track,
                            (SweepArgs<float>*) argbuf);
     else
         hipLaunchKernelGGL(batch_mb_args_kernel<double>,
                            dim3((ngrids + 63)/64), dim3(64), 0, stream, grids,
-                           ngrids, sweep, ts.bx, ts.by, serial, err, fp32,
-                           track, (SweepArgs<double>*) argbuf);
+                           ngrids, sweep, linit, ts.bx, ts.by, serial, err,
+                           fp32, track, (SweepArgs<double>*) argbuf);
     void* args[] = {&argbuf, (void*) &map};
     hipError_t e = hipLaunchKernel(fn, dim3(nwg),
                                    dim3(ts.bx*ts.by + 64*ts.mode), args, 0,

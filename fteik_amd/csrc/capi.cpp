@@ -2450,12 +2450,14 @@ struct Batch3D {
     int *d_map = nullptr, nwg = 0, mapTile = -1;
     unsigned long long *d_flags = nullptr, serial = 0;
     int nflagsCap = 0;
-    // iteration sweeps with mailbox tiles (mbTile >= 0; init sweeps keep
-    // the flag tile): per-grid mailboxes in d_mb, workgroup order d_mbmap,
-    // per-launch sweep arguments d_mbargs
-    // (16 x 16 single-wave-support mailbox tiles by default: 88 ms against
+    // sweeps with mailbox tiles (mbTile >= 0): per-grid mailboxes in d_mb,
+    // the workgroup orders in d_mbmap (the iteration sweeps' first, then the
+    // eight initialization sweeps': entries mbOff[s] .. mbOff[s] + mbN[s] - 1
+    // for s = 0 (iterations), 1..8), per-launch sweep arguments d_mbargs
+    // (16 x 16 single-wave-support mailbox tiles by default: 78 ms against
     // 113 ms for the best flag tile on 64 grids of 64..192 per axis)
-    int mbTile = 7, mapMbTile = -1, mbnwg = 0;
+    int mbTile = 7, mapMbTile = -1;
+    int mbOff[9] = {0}, mbN[9] = {0};
     unsigned long long *d_mb = nullptr;
     size_t mbCap = 0;
     int *d_mbmap = nullptr;
@@ -2550,20 +2552,38 @@ bool batch_prepare(const char *who)
     gb.nwg = nwg;
     gb.mapTile = gb.tile;
     dfree(gb.d_mbmap);
-    gb.mbnwg = 0;
     gb.mapMbTile = -1;
     if (gb.mbTile >= 0) {
-        const int n = fteik_k_batch_mb_map(gb.mbTile, ng, dims.data(), nullptr,
-                                           0);
-        std::vector<int> mbmap(2*(size_t) std::max(n, 1));
-        if (n < 1 || fteik_k_batch_mb_map(gb.mbTile, ng, dims.data(),
-                                          mbmap.data(), n) != n ||
-            !dalloc(&gb.d_mbmap, mbmap.size()) ||
+        std::vector<int> srcs(3*(size_t) ng);
+        for (int i = 0; i < ng; i++) {
+            srcs[3*(size_t) i] = desc[(size_t) i].zsi;
+            srcs[3*(size_t) i + 1] = desc[(size_t) i].xsi;
+            srcs[3*(size_t) i + 2] = desc[(size_t) i].ysi;
+        }
+        std::vector<int> mbmap;
+        for (int sw = 0; sw <= 8; sw++) {
+            const int n = fteik_k_batch_mb_map(gb.mbTile, ng, dims.data(),
+                                               srcs.data(), sw, nullptr, 0);
+            if (n < 0 || (sw == 0 && n < 1)) {
+                msg("%s: empty batch", who);
+                return false;
+            }
+            gb.mbOff[sw] = (int) (mbmap.size()/2);
+            gb.mbN[sw] = n;
+            mbmap.resize(mbmap.size() + 2*(size_t) n);
+            if (n > 0 && fteik_k_batch_mb_map(gb.mbTile, ng, dims.data(),
+                                              srcs.data(), sw,
+                                              mbmap.data() + 2*(size_t)
+                                              gb.mbOff[sw], n) != n) {
+                msg("%s: workgroup order failed", who);
+                return false;
+            }
+        }
+        if (!dalloc(&gb.d_mbmap, mbmap.size()) ||
             !h2d(gb.d_mbmap, mbmap.data(), mbmap.size()*sizeof(int))) {
             msg("%s: GPU error", who);
             return false;
         }
-        gb.mbnwg = n;
     }
     gb.mapMbTile = gb.mbTile;
     return true;
@@ -2571,10 +2591,13 @@ bool batch_prepare(const char *who)
 
 bool batch_sweep(int sweep, bool linit, bool fp32, bool track)
 {
-    if (!linit && gb.mbTile >= 0) {
+    if (gb.mbTile >= 0) {
+        const int m = linit ? sweep : 0;
         const int rc = fteik_k_batch_mb_sweep(gb.d_desc, (int) gb.g.size(),
-                                              gb.d_mbmap, gb.mbnwg, gb.mbTile,
-                                              sweep, ++gb.serial, gb.d_err,
+                                              gb.d_mbmap + 2*(size_t)
+                                              gb.mbOff[m], gb.mbN[m],
+                                              gb.mbTile, sweep, linit ? 1 : 0,
+                                              ++gb.serial, gb.d_err,
                                               fp32 ? 1 : 0, track ? 1 : 0,
                                               gb.d_mbargs, g_dev.stream);
         if (rc != 0) msg("fteik_batch3d_solve: sweep launch failed (%d)", rc);
@@ -2723,8 +2746,8 @@ void fteik_batch3d_setTile(int tile, int *ierr)
         *ierr = 1;
         return;
     }
-    // a flag tile serves every sweep; a mailbox tile serves the iteration
-    // sweeps, the initialization sweeps keep the flag tile
+    // a flag tile (mode 0) or a mailbox tile (modes 1, 2) serves every
+    // sweep
     if (fteik_k_tile_mode(tile) == 0) {
         gb.tile = tile;
         gb.mbTile = -1;

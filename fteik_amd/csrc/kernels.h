@@ -59,13 +59,14 @@ int fteik_k_batch_sweep(const fteik::BatchGrid* grids, const int* map,
                         int nwg, int tile, int sweep, int linit,
                         unsigned long long serial, unsigned long long* flags,
                         int* err, int fp32, int track, hipStream_t stream);
-int fteik_k_batch_mb_map(int tile, int ngrids, const int* dims, int* map,
-                         int map_cap);
+int fteik_k_batch_mb_map(int tile, int ngrids, const int* dims,
+                         const int* srcs, int sweep, int* map, int map_cap);
 size_t fteik_k_batch_mb_args_bytes(int ngrids);
 int fteik_k_batch_mb_sweep(const fteik::BatchGrid* grids, int ngrids,
                            const int* map, int nwg, int tile, int sweep,
-                           unsigned long long serial, int* err, int fp32,
-                           int track, void* argbuf, hipStream_t stream);
+                           int linit, unsigned long long serial, int* err,
+                           int fp32, int track, void* argbuf,
+                           hipStream_t stream);
 // sweep3d.hip
 int fteik_k_sweep3d(const fteik::SweepLaunch* L, int sweep, int linit,
                     hipStream_t stream);

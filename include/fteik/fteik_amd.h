@@ -240,10 +240,8 @@ void fteik_batch3d_setPrecision(int mode, int *ierr);
  * fteik_solver3d_setMixedParameters) */
 void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
                                       int *ierr);
-/* tile shape of the batch (see fteik_solver3d_setTile): a flag tile
- * (mode 0) serves every sweep; a mailbox tile (modes 1, 2) serves the
- * iteration sweeps while the initialization sweeps keep the last flag tile
- * set (defaults: 6 for initialization, mailbox tile 7 for iterations) */
+/* tile shape of the batch (see fteik_solver3d_setTile), flag (mode 0) or
+ * mailbox (modes 1, 2); it serves every sweep (default: mailbox tile 7) */
 void fteik_batch3d_setTile(int tile, int *ierr);
 void fteik_batch3d_solve(int nsweep, double eps, int *ierr);
 void fteik_batch3d_getLastSolveInfo(double *ms, int *corrIterations,

@@ -33,8 +33,7 @@ def _batch(grids, models, nsweep, eps, tile=None, precision=0):
         L.fteik_batch3d_setGrid(g + 1, nz, nx, ny, 0.0, 0.0, 0.0, h, h, h,
                                 _d(v), zc * h, xc * h, yc * h, byref(ierr))
         assert ierr.value == 0
-    # an int, or a sequence: a flag tile for the initialization sweeps
-    # followed by a mailbox tile for the iteration sweeps
+    # an int, or a sequence of tiles set in turn (the last one counts)
     for t in ([] if tile is None else
               [tile] if isinstance(tile, int) else tile):
         L.fteik_batch3d_setTile(t, byref(ierr))
