@@ -595,7 +595,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
               dalloc(&g3.d_mb, 2*(size_t) ngrd) &&
               dalloc(&g3.d_colmod, (size_t) nx*ny) &&
               dalloc(&g3.d_lastproc, 8*(size_t) g3.max_flags) &&
-              dalloc(&g3.d_plan, 1 + 2*(size_t) g3.max_flags) &&
+              dalloc(&g3.d_plan, 1 + 3*(size_t) g3.max_flags) &&
               dalloc(&g3.d_prog, 2*(size_t) g3.max_flags) &&
               dalloc(&g3.d_maxdelta, 1);
     if (!ok) {
@@ -611,6 +611,9 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
                    g_dev.stream);
     hipMemsetAsync(g3.d_prog, 0, 2*(size_t) g3.max_flags*
                    sizeof(unsigned long long), g_dev.stream);
+    // plan: count, order, skip flags and the dirty marks (all clear)
+    hipMemsetAsync(g3.d_plan, 0, (1 + 3*(size_t) g3.max_flags)*sizeof(int),
+                   g_dev.stream);
     hipEventCreate(&g3.ev0);
     hipEventCreate(&g3.ev1);
     // tile shapes whose whole grid of workgroups can be co-resident (FP64

@@ -149,44 +149,40 @@ constexpr int kPlanMaxTiles = 4096;
 // Tiles cover the box [fx0, fx1] x [fy0, fy1] (flipped columns) from its
 // first column; the column row before the box is read by the first tiles
 // and counts as theirs.
-__global__ __launch_bounds__(1024)
-void skip_plan_kernel(const unsigned long long* __restrict__ colmod,
-                      unsigned long long* __restrict__ lastproc,
-                      int* __restrict__ plan, int nx, int ny, int sx, int sy,
+// Dirty marks for the plan below: a tile is marked when one of its columns
+// (or one of the column row before the box, for the first tiles) changed
+// after the tile's last pass in this direction.  One thread per column, so
+// the column stamps are read by the whole GPU rather than one workgroup;
+// concurrent marks of a tile all write 1.
+__global__ __launch_bounds__(256)
+void skip_mark_kernel(const unsigned long long* __restrict__ colmod,
+                      const unsigned long long* __restrict__ lastproc,
+                      int* __restrict__ mark, int nx, int ny, int sx, int sy,
                       int fx0, int fx1, int fy0, int fy1, int bx, int by,
-                      int nbx, int nby, unsigned long long serial)
+                      int nby)
 {
-    __shared__ unsigned long long s_last[kPlanMaxTiles];
+    const int lx = fx1 - fx0 + 2, ly = fy1 - fy0 + 2;
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= lx*ly) return;
+    const int fk = fy0 - 1 + c / lx, fj = fx0 - 1 + c % lx;
+    const int t = (max(fj - fx0, 0)/bx)*nby + max(fk - fy0, 0)/by;
+    const int j = unflip(fj, nx, sx), k = unflip(fk, ny, sy);
+    if (colmod[(long) (k - 1)*nx + (j - 1)] > lastproc[t]) mark[t] = 1;
+}
+
+__global__ __launch_bounds__(1024)
+void skip_plan_kernel(unsigned long long* __restrict__ lastproc,
+                      int* __restrict__ plan, int nbx, int nby,
+                      unsigned long long serial)
+{
     __shared__ unsigned char s_dirty[kPlanMaxTiles];
     const int nt = nbx*nby;
+    // dirty marks of the tiles, set by skip_mark_kernel; cleared here for
+    // the next sweep
+    int* const mark = plan + 1 + 2*nt;
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-        s_last[t] = lastproc[t];
-        s_dirty[t] = 0;
-    }
-    __syncthreads();
-    // every column of the box and the row before it; each thread has PU
-    // stamp loads in flight before it tests any of them
-    constexpr int PU = 8;
-    const int lx = fx1 - fx0 + 2, ly = fy1 - fy0 + 2, ncol = lx*ly;
-    for (int c0 = threadIdx.x; c0 < ncol; c0 += PU*blockDim.x) {
-        unsigned long long m[PU];
-        int tl[PU];
-        #pragma unroll
-        for (int u = 0; u < PU; u++) {
-            const int c = c0 + u*blockDim.x;
-            m[u] = 0;
-            tl[u] = 0;
-            if (c < ncol) {
-                const int fk = fy0 - 1 + c / lx, fj = fx0 - 1 + c % lx;
-                tl[u] = (max(fj - fx0, 0)/bx)*nby + max(fk - fy0, 0)/by;
-                const int j = unflip(fj, nx, sx), k = unflip(fk, ny, sy);
-                m[u] = colmod[(long) (k - 1)*nx + (j - 1)];
-            }
-        }
-        #pragma unroll
-        for (int u = 0; u < PU; u++)
-            if (c0 + u*blockDim.x < ncol && m[u] > s_last[tl[u]])
-                s_dirty[tl[u]] = 1;
+        s_dirty[t] = mark[t] ? 1 : 0;
+        mark[t] = 0;
     }
     __syncthreads();
     for (int p = threadIdx.x; p < nbx; p += blockDim.x) {
@@ -356,11 +352,15 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
                       nblk <= kPlanMaxTiles;
     if (skip) {
         a.colmod = L.colmod; a.plan = L.plan;
+        const int ncol = (a.fx1 - a.fx0 + 2)*(a.fy1 - a.fy0 + 2);
+        hipLaunchKernelGGL(skip_mark_kernel, dim3((ncol + 255)/256),
+                           dim3(256), 0, stream, L.colmod,
+                           L.lastproc + (long) (sweep - 1)*nblk,
+                           L.plan + 1 + 2*nblk, L.nx, L.ny, a.sx, a.sy,
+                           a.fx0, a.fx1, a.fy0, a.fy1, ts.bx, ts.by, a.nby);
         hipLaunchKernelGGL(skip_plan_kernel, dim3(1), dim3(1024), 0, stream,
-                           L.colmod, L.lastproc + (long) (sweep - 1)*nblk,
-                           L.plan, L.nx, L.ny, a.sx, a.sy, a.fx0, a.fx1,
-                           a.fy0, a.fy1, ts.bx, ts.by, a.nbx, a.nby,
-                           L.serial);
+                           L.lastproc + (long) (sweep - 1)*nblk, L.plan,
+                           a.nbx, a.nby, L.serial);
     }
     const void* fn = pencil_fn<R>(linit ? 1 : (skip ? 2 : 0), tile);
     if (L.cooperative)

@@ -100,6 +100,7 @@ struct SweepArgs {
     //   plan[0]           number of tiles processed in this launch
     //   plan[1 + w]       tile of workgroup w (w < plan[0])
     //   plan[1 + ntiles + t]  1 if tile t is skipped in this launch
+    //   plan[1 + 2*ntiles + t]  dirty mark of tile t (plan-building scratch)
     unsigned long long* colmod;
     const int* plan;
     // Sweep chains (consecutive iteration sweeps in one launch; sweep q
