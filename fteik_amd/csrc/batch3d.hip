@@ -49,6 +49,7 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
     a.serial = serial;
     a.maxdelta = track ? g.maxdelta : nullptr;
     a.mb = nullptr;
+    a.mbx = a.mby = nullptr; a.mbnt = 0;
     a.colmod = nullptr; a.plan = nullptr;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) {
         if (threadIdx.x == 0)
@@ -115,6 +116,7 @@ __global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
     a.nby = max((a.fy1 - a.fy0 + by)/by, 0);
     a.maxdelta = track ? g.maxdelta : nullptr;
     a.mb = g.mb;
+    a.mbx = a.mby = nullptr; a.mbnt = 0;   // per-grid per-node mailboxes
     a.colmod = nullptr; a.plan = nullptr; a.prog = nullptr;
     a.pserial = 0; a.par = 0; a.psz = a.psx = a.psy = 0;
     args[i] = a;

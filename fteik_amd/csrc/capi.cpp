@@ -254,6 +254,10 @@ struct Solver3D {
     void *d_cst = nullptr, *d_cst32 = nullptr;
     unsigned long long *d_flags = nullptr;
     unsigned long long *d_mb = nullptr;  // tagged mailbox, 2 words per node
+    // edge-major mailbox of single sweeps: x and y tile edges (at most
+    // (n + 7)/8 + 1 tiles per axis), mbnt = nz + nx + ny steps each
+    unsigned long long *d_mbx = nullptr, *d_mby = nullptr;
+    int mbnt = 0;
     unsigned long long *d_colmod = nullptr, *d_lastproc = nullptr;
     int *d_plan = nullptr;                     // tile skipping state
     bool skipping = true;
@@ -281,7 +285,7 @@ void free3d()
     dfree(g3.d_slow32); dfree(g3.d_tt32);
     dfree(g3.d_cst); dfree(g3.d_cst32);
     dfree(g3.d_flags); dfree(g3.d_err); dfree(g3.d_maxdelta);
-    dfree(g3.d_mb);
+    dfree(g3.d_mb); dfree(g3.d_mbx); dfree(g3.d_mby);
     dfree(g3.d_colmod); dfree(g3.d_lastproc); dfree(g3.d_plan);
     dfree(g3.d_prog);
     dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr);
@@ -319,6 +323,14 @@ SweepLaunch launch3d(bool fp32)
     L.cooperative = g3.cooperative;
     L.tile = fp32 ? g3.tile32 : g3.tile;
     L.mb = g3.d_mb;
+    // the edge-major mailbox, except for the 16-wide tiles without a store
+    // wave (7, 8), which are faster with the per-node one (257^3 FP64 sweep
+    // with 16x8 tiles: 1.14 ms per node, 1.21 ms per edge; 8x8 tiles: 1.11
+    // and 1.03 ms)
+    const bool edge = !(L.tile == 7 || L.tile == 8);
+    L.mbx = edge ? g3.d_mbx : nullptr;
+    L.mby = edge ? g3.d_mby : nullptr;
+    L.mbnt = g3.mbnt;
     L.colmod = nullptr; L.lastproc = nullptr; L.plan = nullptr;
     L.prog = g3.d_prog;
     return L;
@@ -587,12 +599,16 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     const long ngrd = (long) nz*nx*ny;
     const long ncell = (long) (nz - 1)*(nx - 1)*(ny - 1);
     g3.max_flags = fteik_k_max_flags(nx, ny);
+    g3.mbnt = nz + nx + ny;
+    const size_t mbxw = 2*(size_t) ((nx + 7)/8 + 1)*g3.mbnt*ny;
+    const size_t mbyw = 2*(size_t) ((ny + 7)/8 + 1)*g3.mbnt*nx;
     bool ok = dalloc(&g3.d_slow, ncell) && dalloc(&g3.d_tt, ngrd) &&
               dalloc(&g3.d_scr, ngrd) &&
               dalloc((char **) &g3.d_cst, 512) &&
               dalloc((char **) &g3.d_cst32, 512) &&
               dalloc(&g3.d_flags, g3.max_flags) && dalloc(&g3.d_err, 1) &&
               dalloc(&g3.d_mb, 2*(size_t) ngrd) &&
+              dalloc(&g3.d_mbx, mbxw) && dalloc(&g3.d_mby, mbyw) &&
               dalloc(&g3.d_colmod, (size_t) nx*ny) &&
               dalloc(&g3.d_lastproc, 8*(size_t) g3.max_flags) &&
               dalloc(&g3.d_plan, 1 + 3*(size_t) g3.max_flags) &&
@@ -609,6 +625,10 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     hipMemsetAsync(g3.d_err, 0, sizeof(int), g_dev.stream);
     hipMemsetAsync(g3.d_mb, 0, 2*(size_t) ngrd*sizeof(unsigned long long),
                    g_dev.stream);
+    hipMemsetAsync(g3.d_mbx, 0, mbxw*sizeof(unsigned long long),
+                   g_dev.stream);
+    hipMemsetAsync(g3.d_mby, 0, mbyw*sizeof(unsigned long long),
+                   g_dev.stream);
     hipMemsetAsync(g3.d_prog, 0, 2*(size_t) g3.max_flags*
                    sizeof(unsigned long long), g_dev.stream);
     // plan: count, order, skip flags and the dirty marks (all clear)
@@ -624,10 +644,10 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
         g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
                                         fteik_k_pencil_fits(nx, ny, 1, t));
     // default: the first available shape in order of measured speed (FP64
-    // sweep of a 257^3 grid: 8x8 mailbox tiles 1.10 ms, 16x8 1.14 ms, 16x8
-    // with a store wave 1.15 ms); on smaller grids the 8x8 tiles keep more
-    // CUs busy as well.
-    static const int pref_large[] = {9, 8, 11, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
+    // sweep of a 257^3 grid: 8x8 mailbox tiles 1.03 ms, 16x8 with a store
+    // wave 1.10 ms, 16x8 without 1.14 ms); on smaller grids the 8x8 tiles
+    // keep more CUs busy as well.
+    static const int pref_large[] = {9, 11, 8, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
     static const int pref_small[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
     const bool large = (long) (nx - 1)*(ny - 1) >= 128L*g_dev.ncu;
     int first = -1;

@@ -29,6 +29,11 @@ struct SweepLaunch {
     int cooperative;
     int tile;                    // tile shape index of the pipelined sweep
     unsigned long long* mb;      // mailbox of the mode-1 tiles
+    // edge-major mailbox of sweeps outside chains (null: per-node mailbox):
+    // x edges (nx + 7)/8 + 1 tiles by mbnt steps by ny, y edges likewise
+    unsigned long long* mbx;
+    unsigned long long* mby;
+    int mbnt;
     // tile skipping state (null: off): per-column serial of the last change
     // (nx*ny), per-direction serial of each tile's last pass (8*max_flags),
     // and the launch plan (1 + 2*max_flags ints)

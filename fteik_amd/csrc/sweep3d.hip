@@ -315,6 +315,7 @@ static int launch_sweep_t(const SweepLaunch& L, int sweep, bool linit,
     a.fy0 = b[4]; a.fy1 = b[5];
     a.maxdelta = L.maxdelta;
     a.mb = L.mb;
+    a.mbx = L.mbx; a.mby = L.mby; a.mbnt = L.mbnt;
     a.colmod = nullptr; a.plan = nullptr;
     if (a.fz0 > a.fz1 || a.fx0 > a.fx1 || a.fy0 > a.fy1) return 0;
     if (L.schedule == 1) {
@@ -464,6 +465,8 @@ static int launch_chain_t(const SweepLaunch& L, int dir0, int nq,
     if (a.nbx*a.nby > L.max_flags) return 2;
     a.maxdelta = L.maxdelta;
     a.mb = L.mb;
+    a.mbx = a.mby = nullptr;   // chains keep the per-node mailbox
+    a.mbnt = 0;
     a.prog = L.prog;
     void* args[] = {&a, &dir0};
     hipError_t e = hipLaunchKernel(fn, dim3(nq*a.nbx*a.nby),

@@ -95,6 +95,16 @@ struct SweepArgs {
     int nbx, nby;                // blocks along flipped x, y
     double* maxdelta;            // optional: max |t_new - t_old| (as bits)
     unsigned long long* mb;      // mailbox (mode 1): 2 words/node FP64, 1 FP32
+    // Edge-major mailbox (sweeps outside chains; null: the per-node one
+    // above).  The last column of x-tile ex posts its node of row fk at
+    // step t as entry (ex*mbnt + t)*ny + fk of mbx, the last row of y-tile
+    // ey its node of column fj as entry (ey*mbnt + t)*nx + fj of mby: the
+    // values one halo edge reads at a step are contiguous, a few cache
+    // lines instead of one per column.  Entries have the per-node layout's
+    // size (2 words FP64, 1 FP32).
+    unsigned long long* mbx;
+    unsigned long long* mby;
+    int mbnt;                    // steps per tile edge (> the last step)
     // Tile skipping (mailbox tiles, iteration sweeps; null = off):
     //   colmod[k*nx + j]  launch serial of the last change of column (j, k)
     //   plan[0]           number of tiles processed in this launch
@@ -584,7 +594,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
         // are computed once
         const int sl = threadIdx.x - NC - 64;
         long sbase[NW];
-        int sdep[NW], sring[NW], sflag[NW];
+        int sdep[NW], sring[NW], sflag[NW], sfj[NW], sfk[NW];
         #pragma unroll
         for (int c = 0; c < NW; c++) {
             const int cix = sl + 64*c;
@@ -593,8 +603,11 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             const bool colE = (fj >= jlo && fj <= jhi && fk >= klo &&
                                fk <= khi);
             const bool post = (lj == BX - 1 || lk == BY - 1);
-            sflag[c] = colE ? (1 | (post ? 2 : 0)) : 0;
+            sflag[c] = colE ? (1 | (post ? 2 : 0) | (lj == BX - 1 ? 4 : 0)
+                               | (lk == BY - 1 ? 8 : 0)) : 0;
             sdep[c] = fj + fk;
+            sfj[c] = fj;
+            sfk[c] = fk;
             sring[c] = (lk + 1)*(BX + 1) + lj + 1;
             sbase[c] = colE ? (long) (unflip(fk, ny, d.sy) - 1)*nz*nx
                               + (long) (unflip(fj, nx, d.sx) - 1)*nz : 0;
@@ -617,7 +630,18 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                     if (CHAIN) st_coh(a.tt + g, v);
                     else a.tt[g] = v;
                 }
-                if (post) mb_put(a.mb, g, v, tag);
+                if (post) {
+                    if (!CHAIN && a.mbx) {
+                        if (sflag[c] & 4)
+                            mb_put(a.mbx, ((long) fbx*a.mbnt + t)*ny + sfk[c],
+                                   v, tag);
+                        if (sflag[c] & 8)
+                            mb_put(a.mby, ((long) fby*a.mbnt + t)*nx + sfj[c],
+                                   v, tag);
+                    } else {
+                        mb_put(a.mb, g, v, tag);
+                    }
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
             if (sl == 0)
@@ -777,9 +801,22 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 if (hcol && fi >= ez0 && fi <= a.fz1) {
                     R v = cur[q];
                     if (!direct) {
-                        const long g = gcol + ((d.sz > 0) ? fi : nz - 1 - fi);
+                        unsigned long long* mbp = a.mb;
+                        long g = gcol + ((d.sz > 0) ? fi : nz - 1 - fi);
+                        if (!CHAIN && a.mbx) {
+                            // x-side lanes and the corner read the last
+                            // column of x-tile fbx - 1, the others the last
+                            // row of y-tile fby - 1
+                            if (hl < BY || hl == BX + BY) {
+                                mbp = a.mbx;
+                                g = ((long) (fbx - 1)*a.mbnt + s)*ny + hk;
+                            } else {
+                                mbp = a.mby;
+                                g = ((long) (fby - 1)*a.mbnt + s)*nx + hj;
+                            }
+                        }
                         spins = 0;
-                        while (!mb_get(a.mb, g, tag, v)) {
+                        while (!mb_get(mbp, g, tag, v)) {
                             __builtin_amdgcn_s_sleep(1);
                             if (++spins > kSpinLimit) {
                                 atomicExch(a.err, 1);
@@ -928,7 +965,18 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                     v = vn;
                     if (!SW) a.tt[colbase + node_off(fi)] = v;
                 }
-                if (!SW && post) mb_put(a.mb, colbase + node_off(fi), v, tag);
+                if (!SW && post) {
+                    if (!CHAIN && a.mbx) {
+                        if (lj == BX - 1)
+                            mb_put(a.mbx, ((long) fbx*a.mbnt + t)*ny + fk, v,
+                                   tag);
+                        if (lk == BY - 1)
+                            mb_put(a.mby, ((long) fby*a.mbnt + t)*nx + fj, v,
+                                   tag);
+                    } else {
+                        mb_put(a.mb, colbase + node_off(fi), v, tag);
+                    }
+                }
                 ring[t & (RING - 1)][me] = v;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
