@@ -5,7 +5,8 @@
 // candidate cells that the exact tables score.
 //
 // This has no counterpart in the reference (locate.F90 only searches the
-// full grid); the full grid search stays the fallback and the judge.
+// full grid); the exact full grid search stays the fallback and the
+// reference result.
 //
 // Layout: activations are row-major [rows][features]; weights W_l are
 // row-major [out][in] (FP32 master copy, BF16 copy for the MFMA GEMMs).
