@@ -116,7 +116,7 @@ __global__ void batch_mb_args_kernel(const BatchGrid* __restrict__ grids,
     a.nby = max((a.fy1 - a.fy0 + by)/by, 0);
     a.maxdelta = track ? g.maxdelta : nullptr;
     a.mb = g.mb;
-    a.mbx = a.mby = nullptr; a.mbnt = 0;   // per-grid per-node mailboxes
+    a.mbx = g.mbx; a.mby = g.mby; a.mbnt = g.mbnt;
     a.colmod = nullptr; a.plan = nullptr; a.prog = nullptr;
     a.pserial = 0; a.par = 0; a.psz = a.psx = a.psy = 0;
     args[i] = a;

@@ -2477,12 +2477,16 @@ struct Batch3D {
     // the workgroup orders in d_mbmap (the iteration sweeps' first, then the
     // eight initialization sweeps': entries mbOff[s] .. mbOff[s] + mbN[s] - 1
     // for s = 0 (iterations), 1..8), per-launch sweep arguments d_mbargs
-    // (16 x 16 single-wave-support mailbox tiles by default: 78 ms against
-    // 113 ms for the best flag tile on 64 grids of 64..192 per axis)
-    int mbTile = 7, mapMbTile = -1;
+    // (8 x 8 store-wave mailbox tiles with edge-major mailboxes by default:
+    // 70 ms against 78 ms for 16 x 16 tiles and 113 ms for the best flag
+    // tile on 64 grids of 64..192 per axis)
+    int mbTile = 12, mapMbTile = -1;
     int mbOff[9] = {0}, mbN[9] = {0};
     unsigned long long *d_mb = nullptr;
     size_t mbCap = 0;
+    // edge-major mailboxes of the grids (mailbox tiles other than 7, 8)
+    unsigned long long *d_mbe = nullptr;
+    size_t mbeCap = 0;
     int *d_mbmap = nullptr;
     char *d_mbargs = nullptr;
     int *d_err = nullptr;
@@ -2500,7 +2504,7 @@ void free_batch()
         dfree(x.d_cst); dfree(x.d_cst32);
     }
     dfree(gb.d_desc); dfree(gb.d_map); dfree(gb.d_flags); dfree(gb.d_err);
-    dfree(gb.d_maxdelta); dfree(gb.d_mb); dfree(gb.d_mbmap);
+    dfree(gb.d_maxdelta); dfree(gb.d_mb); dfree(gb.d_mbmap); dfree(gb.d_mbe);
     dfree(gb.d_mbargs);
     if (gb.ev0) hipEventDestroy(gb.ev0);
     if (gb.ev1) hipEventDestroy(gb.ev1);
@@ -2534,7 +2538,31 @@ bool batch_prepare(const char *who)
             return false;
         }
     }
-    size_t mboff = 0;
+    // edge-major mailboxes (as the single-grid solver: not for the 16-wide
+    // tiles without a store wave)
+    const bool edge = gb.mbTile >= 0 && gb.mbTile != 7 && gb.mbTile != 8;
+    auto edge_words = [](const BGrid &x, bool ydir) -> size_t {
+        const int nt = x.nz + x.nx + x.ny;
+        return ydir ? 2*(size_t) ((x.ny + 7)/8 + 1)*nt*x.nx
+                    : 2*(size_t) ((x.nx + 7)/8 + 1)*nt*x.ny;
+    };
+    if (edge) {
+        size_t words = 0;
+        for (const BGrid &x : gb.g) words += edge_words(x, false)
+                                           + edge_words(x, true);
+        if (words > gb.mbeCap) {
+            dfree(gb.d_mbe);
+            gb.mbeCap = 0;
+            if (!dalloc(&gb.d_mbe, words)) {
+                msg("%s: out of GPU memory", who);
+                return false;
+            }
+            gb.mbeCap = words;
+            hipMemsetAsync(gb.d_mbe, 0, words*sizeof(unsigned long long),
+                           g_dev.stream);
+        }
+    }
+    size_t mboff = 0, mbeoff = 0;
     for (int i = 0; i < ng; i++) {
         const BGrid &x = gb.g[(size_t) i];
         fteik::BatchGrid &d = desc[(size_t) i];
@@ -2543,6 +2571,14 @@ bool batch_prepare(const char *who)
         d.maxdelta = gb.d_maxdelta + i;
         d.mb = gb.mbTile >= 0 ? gb.d_mb + mboff : nullptr;
         mboff += 2*(size_t) x.nz*x.nx*x.ny;
+        d.mbx = d.mby = nullptr;
+        d.mbnt = x.nz + x.nx + x.ny;
+        if (edge) {
+            d.mbx = gb.d_mbe + mbeoff;
+            mbeoff += edge_words(x, false);
+            d.mby = gb.d_mbe + mbeoff;
+            mbeoff += edge_words(x, true);
+        }
         d.nz = x.nz; d.nx = x.nx; d.ny = x.ny;
         d.zsi = x.src.zsi; d.xsi = x.src.xsi; d.ysi = x.src.ysi;
         dims[3*(size_t) i] = x.nz; dims[3*(size_t) i + 1] = x.nx;

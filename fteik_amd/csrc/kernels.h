@@ -51,6 +51,9 @@ struct BatchGrid {
     void* tt32; const void* slow32; const void* cst32;  // FP32 (mixed)
     double* maxdelta;                                   // per-grid monitor
     unsigned long long* mb;        // per-grid mailbox (mailbox-tile sweeps)
+    unsigned long long* mbx;       // per-grid edge-major mailbox (null: the
+    unsigned long long* mby;       // per-node one), as SweepLaunch::mbx/mby
+    int mbnt;
     int nz, nx, ny, zsi, xsi, ysi;
 };
 

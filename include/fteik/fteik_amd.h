@@ -241,7 +241,7 @@ void fteik_batch3d_setPrecision(int mode, int *ierr);
 void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
                                       int *ierr);
 /* tile shape of the batch (see fteik_solver3d_setTile), flag (mode 0) or
- * mailbox (modes 1, 2); it serves every sweep (default: mailbox tile 7) */
+ * mailbox (modes 1, 2); it serves every sweep (default: mailbox tile 12) */
 void fteik_batch3d_setTile(int tile, int *ierr);
 void fteik_batch3d_solve(int nsweep, double eps, int *ierr);
 void fteik_batch3d_getLastSolveInfo(double *ms, int *corrIterations,
