@@ -149,6 +149,74 @@ def locator_workload():
     return (nz, nx, ny, h), vel, stations
 
 
+def solver2d_section(L, nsrc=256, cpu_sources=4):
+    """2D solver, many sources in one launch (one workgroup per source):
+    501 x 501 nodes, h = 10 m, v = 1500 + 0.8 z times a +-5 % smooth
+    perturbation; 256 sources at seeded random nodes (seed 21), 32 surface
+    receivers; nsweep = 2, eps = 3 cells.  Parity: the receiver times of a
+    sample of sources equal the oracle's; CPU baseline: the oracle (one
+    thread) on the same sample."""
+    nz, nx, h = 501, 501, 10.0
+    zc = (np.arange(nz - 1) + 0.5) * h
+    xc = (np.arange(nx - 1) + 0.5) * h
+    pert = 1.0 + 0.05 * np.sin(xc[:, None] / 700.0) * np.cos(zc[None, :] / 450.0)
+    vel = np.ascontiguousarray(((1500.0 + 0.8 * zc)[None, :] * pert).ravel())
+    rng = np.random.default_rng(21)
+    zs = rng.uniform(0.0, (nz - 2) * h, nsrc)
+    xs = rng.uniform(0.0, (nx - 2) * h, nsrc)
+    zr = np.zeros(32)
+    xr = np.linspace(0.0, (nx - 1) * h, 32)
+    dp = ctypes.POINTER(c_double)
+    ierr = c_int(1)
+    L.fteik_solver2d_initialize(nz, nx, 0.0, 0.0, h, h, 2, 3.0, 0.0, 0,
+                                byref(ierr))
+    assert ierr.value == 0, "2D initialize"
+    L.fteik_solver2d_setCellVelocityModel64f(vel.size, 0, vel.ctypes.data_as(
+        dp), byref(ierr))
+    L.fteik_solver2d_setSources64f(nsrc, zs.ctypes.data_as(dp),
+                                   xs.ctypes.data_as(dp), byref(ierr))
+    L.fteik_solver2d_setReceivers64f(zr.size, zr.ctypes.data_as(dp),
+                                     xr.ctypes.data_as(dp), byref(ierr))
+    assert ierr.value == 0, "2D set-up"
+    L.fteik_solver2d_solveLSM(byref(ierr))          # warm-up
+    assert ierr.value == 0, "2D solve"
+    reps = 3
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.fteik_solver2d_solveLSM(byref(ierr))
+    hip_sync()
+    sec = (time.perf_counter() - t0) / reps
+    assert ierr.value == 0, "2D solve"
+    ttr = np.zeros(zr.size * nsrc)
+    L.fteik_solver2d_getTravelTimes64f(zr.size, ttr.ctypes.data_as(dp),
+                                       byref(ierr))
+    ttr = ttr.reshape(nsrc, zr.size)
+    # nominal updates: initialization plus nsweep iterations, 4 sweeps over
+    # the (nz-1)(nx-1) updated nodes each
+    upd = nsrc * 3 * 4 * (nz - 1) * (nx - 1)
+    out = {"workload": "2D, 501x501 h=10 m, v=1500+0.8z +-5% smooth, 256 "
+                       "sources (one launch), nsweep=2, eps=3",
+           "sources": nsrc, "seconds": sec, "value": nsrc / sec,
+           "unit": "sources/s", "nominal_Gupdates_per_s": upd / sec / 1e9}
+    from oracle import oracle as orc
+    slow = orc.cell_slowness(vel, nz, nx)
+    grid = (nz, nx, 1, h, h, 1.0, 0.0, 0.0, 0.0)
+    same = True
+    t0 = time.perf_counter()
+    for k in range(cpu_sources):
+        src = orc.source_locate(grid, zs[k], xs[k], slow=slow, is3d=False)
+        ref = orc.solve2d(slow, nz, nx, h, h, src, 2, 3.0)
+        same = same and bool(np.array_equal(
+            ttr[k], orc.receivers(ref, grid, zr, xr, is3d=False)))
+    tcpu = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": cpu_sources / tcpu, "unit": "sources/s",
+                           "cores": 1, "kind": "port",
+                           "sample": "%d sources" % cpu_sources}
+    out["gpu_equals_oracle_on_sample"] = same
+    return out
+
+
 def locator_section(L, args, nev=10000, cpu_events=16):
     """Full grid search over 48 FP32 station tables built by the batched
     solver and handed to the locator on the device."""
@@ -508,6 +576,9 @@ def main():
               file=sys.stderr, flush=True)
         out["locator"] = locator_section(L, args)
         print("bench: locator %.0f events/s" % out["locator"]["value"],
+              file=sys.stderr, flush=True)
+        out["solver2d"] = solver2d_section(L)
+        print("bench: 2D %.0f sources/s" % out["solver2d"]["value"],
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
