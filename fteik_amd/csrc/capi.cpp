@@ -1030,6 +1030,8 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
                 for (int sw = 1; sw <= 8; sw++)
                     updates += sweep_updates(sw, false, s);
         } else {
+            // FP32 iterations skip unchanged tiles as the FP64 ones do
+            ok = ok && reset_skipping(L32);
             for (int k = 0; k < g3.nsweep && ok; k++)
                 for (int sw = 1; sw <= 8 && ok; sw++) {
                     ok = run_sweep(L32, sw, false);
