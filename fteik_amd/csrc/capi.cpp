@@ -241,7 +241,8 @@ struct Solver3D {
            convTol = 0;
     int precision = 0, schedule = 0, cooperative = 1;
     int tile = 0, tile32 = 0;            // pipelined-sweep tile shapes
-    std::vector<char> tileOK;            // tile fits co-resident (both types)
+    std::vector<char> tileOK;            // tile usable on this grid
+    std::vector<char> tileCoRes;         // whole grid of tiles co-resident
     double mixedBias = 1.0/64.0;         // 2^-6 relative promotion margin
     int maxCorr = 64;
     std::vector<SrcInfo> src;
@@ -320,7 +321,14 @@ SweepLaunch launch3d(bool fp32)
     L.nz = g3.nz; L.nx = g3.nx; L.ny = g3.ny;
     L.fp32 = fp32 ? 1 : 0;
     L.schedule = g3.schedule;
-    L.cooperative = g3.cooperative;
+    // a cooperative launch when every tile is resident at once; mailbox
+    // tiles also run without (a tile waits only on lower-numbered ones,
+    // and dispatch is in order)
+    {
+        const int t = fp32 ? g3.tile32 : g3.tile;
+        L.cooperative = (t >= 0 && t < (int) g3.tileCoRes.size() &&
+                         g3.tileCoRes[(size_t) t]) ? 1 : 0;
+    }
     L.tile = fp32 ? g3.tile32 : g3.tile;
     L.mb = g3.d_mb;
     // the edge-major mailbox, except for the 16-wide tiles without a store
@@ -640,14 +648,20 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     // and FP32 kernels); the first such one is the default
     const int ntile = fteik_k_num_tiles();
     g3.tileOK.assign((size_t) ntile, 0);
-    for (int t = 0; t < ntile; t++)
-        g3.tileOK[(size_t) t] = (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
-                                        fteik_k_pencil_fits(nx, ny, 1, t));
+    g3.tileCoRes.assign((size_t) ntile, 0);
+    for (int t = 0; t < ntile; t++) {
+        g3.tileCoRes[(size_t) t] =
+            (char) (fteik_k_pencil_fits(nx, ny, 0, t) &&
+                    fteik_k_pencil_fits(nx, ny, 1, t));
+        g3.tileOK[(size_t) t] = (char) (g3.tileCoRes[(size_t) t] ||
+                                        fteik_k_tile_mode(t) >= 1);
+    }
     // default: the first available shape in order of measured speed (FP64
-    // sweep of a 257^3 grid: 8x8 mailbox tiles 1.03 ms, 16x8 with a store
-    // wave 1.10 ms, 16x8 without 1.14 ms); on smaller grids the 8x8 tiles
+    // sweep of a 257^3 grid: 8x8 tiles with a store wave 0.92 ms, launched
+    // without co-residency; 8x8 without a store wave 1.03 ms; 16x8 with a
+    // store wave 1.10 ms, without 1.14 ms); on smaller grids the 8x8 tiles
     // keep more CUs busy as well.
-    static const int pref_large[] = {9, 11, 8, 12, 7, 10, 6, 3, 1, 0, 5, 2, 4};
+    static const int pref_large[] = {12, 9, 11, 8, 7, 10, 6, 3, 1, 0, 5, 2, 4};
     static const int pref_small[] = {12, 9, 11, 8, 10, 7, 6, 3, 1, 0, 5, 2, 4};
     const bool large = (long) (nx - 1)*(ny - 1) >= 128L*g_dev.ncu;
     int first = -1;

@@ -279,6 +279,32 @@ def test_every_tile_shape_matches_oracle(precision):
     assert tested == nt.value          # every shape fits this grid
 
 
+@pytest.mark.parametrize("skipping", [0, 1])
+def test_tiles_launched_without_coresidency(skipping):
+    """On a 129 x 257 x 257 grid the 1024 workgroups of the 8 x 8
+    store-wave tiles (3 waves each) are not all resident at once: the
+    sweeps launch them without cooperation, in tile order.  The field equals
+    the oracle's and the cooperative 8 x 8 tiles' bit for bit, with and
+    without tile skipping."""
+    nz, nx, ny, h = 129, 257, 257, 20.0
+    vel = hetero(nz, nx, ny, h, 12)
+    zs, xs, ys = 60.3 * h, 100.7 * h, 200.2 * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, 1, 5.0)
+    g = GPU3D(nz, nx, ny, h, nsweep=1)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    L = lib()
+    ierr = c_int(1)
+    L.fteik_solver3d_setSweepOverlap(0, byref(ierr))
+    L.fteik_solver3d_setTileSkipping(skipping, byref(ierr))
+    for tile in (12, 9):
+        _announce("large grid tile", tile, "skipping", skipping)
+        L.fteik_solver3d_setTile(tile, tile, byref(ierr))
+        assert ierr.value == 0
+        assert g.solve(1) == 0
+        np.testing.assert_array_equal(g.field(), ref)
+
+
 def test_tile_skipping_is_exact():
     """Many iterations (so later ones skip most tiles): the field equals the
     oracle's, and equals the solver's own dense result, bit for bit."""
