@@ -1,0 +1,116 @@
+// capi_internal.h -- host-side state shared by the C-ABI source files
+// (capi_*.cpp).  Not installed.  Each capi file implements one group of the
+// entry points of include/fteik/fteik_amd.h; host code only validates
+// inputs, computes scalar geometry (grid positions of sources and
+// receivers), moves data and launches the kernels declared in kernels.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <float.h>
+#include <vector>
+#include <algorithm>
+#include "kernels.h"
+#include "../../include/fteik/fteik_amd.h"
+
+namespace fteik_capi {
+
+using fteik::SourceGeom;
+using fteik::SweepLaunch;
+
+// Message to stderr with a trailing newline (the reference writes its
+// errors to ERROR_UNIT).
+void msg(const char *fmt, ...);
+
+// The one GPU of the process and the library's stream.
+struct Device {
+    bool ready = false;
+    bool failed = false;
+    hipStream_t stream = nullptr;
+    int ncu = 0;
+    int dev = 0;
+};
+extern Device g_dev;
+
+// Opens the device on first use (and registers the exit-time teardown);
+// false, with a message naming `who`, when no GPU is usable.
+bool device_ok(const char *who);
+
+template <typename T>
+inline bool dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **) p, n*sizeof(T)) == hipSuccess;
+}
+
+template <typename T>
+inline void dfree(T *&p)
+{
+    if (p) hipFree((void *) p);
+    p = nullptr;
+}
+
+bool h2d(void *d, const void *h, size_t bytes);
+bool d2h(void *h, const void *d, size_t bytes);
+bool sync_ok();
+double vmin(const double *v, long n);
+
+// Grid position of a source (source.F90:136-190).
+struct SrcInfo {
+    double ztrue, xtrue, ytrue;
+    double zsa, xsa, ysa;
+    int zsi, xsi, ysi;
+};
+
+int locate_source(bool is3d, int nz, int nx, int ny, double dz, double dx,
+                  double dy, double z0, double x0, double y0, double zs,
+                  double xs, double ys, int isrc, SrcInfo *o);
+int locate_receivers(bool is3d, int nz, int nx, int ny, double dz, double dx,
+                     double dy, double z0, double x0, double y0, int nrec,
+                     const double *z, const double *x, const double *y,
+                     std::vector<int> &ri, std::vector<double> &rd);
+bool set_model(const char *who, bool is3d, int nz, int nx, int ny, int kind,
+               int n, int order, const double *vel, double *d_slow);
+bool check_eps(const char *who, double eps, int nz, int nx, int ny,
+               bool is3d);
+
+// The reference's model module state (model.F90:43, :181-216, :615-636):
+// one grid geometry per process, set by whichever solver initialized last
+// (ny = 1 for the 2D solver) and reset to -1 by either solver's free.
+struct ModelGeom {
+    int nz = -1, nx = -1, ny = -1;
+    bool is3d = true;
+};
+extern ModelGeom g_model;
+void model_set(bool is3d, int nz, int nx, int ny);
+void model_free();
+
+// ---------------------------------------------------------------- locator
+// Shared by the locator entry points, the batch-to-table hand-off and the
+// surrogate.
+struct Locator {
+    bool init = false;
+    int nEvents = 0, ntf = 0, ngrd = 0, lntf = 0;
+    long ldgrd = 0;
+    int refCompat = 1;
+    std::vector<double> tepoch;
+    std::vector<float> obs, wt;
+    std::vector<unsigned char> skip;
+    float *d_tab = nullptr, *d_t0 = nullptr, *d_obs = nullptr,
+          *d_wt = nullptr, *d_xn = nullptr;
+    unsigned char *d_skip = nullptr;
+    unsigned long long *d_best = nullptr;
+    double *d_tmp = nullptr;
+};
+extern Locator gl;
+
+int padlength32(int alignment, int n);
+float weight_norm(const float *w, int lntf);
+void mask_skipped(float *o, float *w, float *w2, const unsigned char *sk,
+                  size_t n);
+
+} // namespace fteik_capi

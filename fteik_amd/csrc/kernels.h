@@ -1,5 +1,5 @@
 // kernels.h -- internal launcher interface between the host C-ABI layer
-// (capi.cpp) and the HIP kernels.  Not installed; plain C structs only.
+// (capi_*.cpp) and the HIP kernels.  Not installed; plain C structs only.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -94,16 +94,23 @@ void fteik_solver3d_setPrecision(int mode, int *ierr);
  * 1 = one kernel launch per level (diagnostic schedule). */
 void fteik_solver3d_setSchedule(int schedule, int *ierr);
 /* EXTENSION: mixed-mode parameters: the single-precision field is raised
- * by the relative margin bias (default 2^-8) when it is promoted, so it
- * stays an upper bound of the double-precision solution and the double
- * correction iterations (at most maxCorrections, default 64) converge onto
- * it.  Reset to the defaults by fteik_solver3d_initialize. */
+ * by the relative margin bias (default 2^-6, 0 < bias < 0.1) when it is
+ * promoted, so it stays an upper bound of the double-precision solution and
+ * the double correction iterations (at most maxCorrections, default 64)
+ * converge onto it.  A correction phase that reaches maxCorrections without
+ * converging fails the solve with ierr = 1.  Reset to the defaults by
+ * fteik_solver3d_initialize. */
 void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
                                        int *ierr);
 /* EXTENSION: tile shape of the pipelined FP64 and FP32 sweeps (index into
- * the library's table of workgroup shapes and halo batch sizes; see
- * fteik_solver3d_autotune).  Only shapes whose workgroups can all be
- * resident for this grid are accepted. */
+ * the library's table of workgroup shapes and halo batch sizes; chosen by
+ * fteik_solver3d_autotune, or from a measured preference list at
+ * initialize).  Flag tiles (mode 0) are accepted only when all their
+ * workgroups can be resident; mailbox tiles (modes 1, 2) also launch
+ * without co-residency: a tile waits only on lower-numbered tiles and
+ * workgroups are dispatched in order, so the lowest unfinished tile is
+ * always resident.  Every wait has a spin limit; a sweep that exceeds it
+ * fails the solve with ierr = 1 rather than hanging. */
 void fteik_solver3d_setTile(int tile, int tile32, int *ierr);
 void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
 /* EXTENSION: skip, in the Gauss-Seidel iterations, every tile of the
@@ -300,7 +307,8 @@ void fteik_amd_invHiddenBf16(int R, const float *hin, const float *W,
 int fteik_model_grid2indexF(int i, int j, int k, int nz, int nzx);
 /* model.F90:827 */
 int fteik_model_velGrid2indexF(int i, int j, int k, int nzm1, int nzm1_nxm1);
-/* model.F90:794 (uses the 3D solver's grid) */
+/* model.F90:794 (on the geometry of the solver initialized last, the
+ * reference's shared model module; ny = 1 after fteik_solver2d_initialize) */
 void fteik_model_index2gridF(int igrd, int *i, int *j, int *k, int *ierr);
 /* fwrapper.c:8: 0-based */
 int fteik_model_grid2index(const int iz, const int ix, const int iy,
@@ -374,6 +382,11 @@ void fteik_amd_timeSweeps3d(int precision, int nrep, double *msPerLaunch,
                             int *ierr);
 /* EXTENSION: library build string. */
 const char *fteik_amd_version(void);
+/* EXTENSION: releases every solver's device state, then the library's HIP
+ * stream.  Registered to run at process exit when the GPU is first opened;
+ * a caller may also call it, after which the next call that needs the GPU
+ * opens it again. */
+void fteik_amd_finalize(void);
 
 #ifdef __cplusplus
 }
