@@ -88,8 +88,22 @@ def load_library(path=None):
         "fteik_solver3d_setChainTile": [i, i, _ip],
         "fteik_solver3d_getSweepOverlap": [_ip, _ip, _ip, _ip],
         "fteik_solver3d_getTile": [_ip, _ip, _ip, _ip],
+        "fteik_solver3d_autotune": [i, _ip, _ip, _ip],
+        "fteik_batch3d_autotune": [i, d, _ip, _ip],
         "fteik_solver3d_getLastSolveInfo": [_dp, _ip, _dp, _ip],
         "fteik_solver3d_free": [],
+        "fteik_amd_finalize": [],
+        "fteik_solver3d_traceRaysToPoints": [i, _dp, _dp, _dp, _ip],
+        "fteik_solver3d_traceRaysToReceivers": [_ip],
+        "fteik_solver3d_getTravelTimeGradient64f": [i, _dp, _ip],
+        "fteik_solver2d_traceRaysToPoints": [i, _dp, _dp, _ip],
+        "fteik_solver2d_traceRaysToReceivers": [_ip],
+        "fteik_solver2d_getTravelTimeGradient64f": [i, _dp, _dp, _ip],
+        "fteik_rays_setODEIntegrator": [i, _ip],
+        "fteik_rays_getNumberOfRays": [_ip, _ip],
+        "fteik_rays_getTravelTimes64f": [i, _dp, _ip],
+        "fteik_rays_getRayPath": [i, i, _dp, _ip, _dp, _ip],
+        "fteik_model_index2gridF": [i, _ip, _ip, _ip, _ip],
         "fteik_solver2d_initialize": [i, i, d, d, d, d, i, d, d, i, _ip],
         "fteik_solver2d_setVelocityModel64f": [i, _dp, _ip],
         "fteik_solver2d_setNodalVelocityModel64f": [i, i, _dp, _ip],
@@ -153,6 +167,7 @@ def load_library(path=None):
         "fteik_surrogate_forward": [i, _fp, _fp, _ip],
         "fteik_surrogate_gradients": [i, _fp, _fp, _fp, _dp, _ip],
         "fteik_surrogate_train": [i, i, d, i, _dp, _ip],
+        "fteik_surrogate_setNetworkProposals": [i, _ip],
         "locate_locateEventsSurrogate": [i, _ip, i, i, _ip, _dp, _dp, _ip,
                                          _ip],
         "fteik_amd_gemmBf16": [i, i, i, _fp, _fp, _fp, _ip],

@@ -13,6 +13,7 @@ struct Solver2D {
     std::vector<SrcInfo> src;
     std::vector<int> ri;
     std::vector<double> rd;
+    std::vector<double> zr, xr;      // receiver positions (model coords)
     int nrec = 0;
     std::vector<double> ttimesRec;   // [nrec x nsrc]
     double *d_slow = nullptr, *d_tt = nullptr, *d_scr = nullptr;
@@ -22,6 +23,8 @@ struct Solver2D {
     double *d_rd = nullptr, *d_ttr = nullptr;
     double *d_sa = nullptr;          // zsa, xsa per source
     int *d_si = nullptr;             // zsi, xsi per source
+    double *d_gx = nullptr, *d_gz = nullptr;   // gradient, on demand
+    bool haveGrad = false;
 };
 Solver2D g2;
 
@@ -29,7 +32,7 @@ void free2d()
 {
     dfree(g2.d_slow); dfree(g2.d_tt); dfree(g2.d_scr); dfree(g2.d_ttb);
     dfree(g2.d_ri); dfree(g2.d_rd); dfree(g2.d_ttr);
-    dfree(g2.d_sa); dfree(g2.d_si);
+    dfree(g2.d_sa); dfree(g2.d_si); dfree(g2.d_gx); dfree(g2.d_gz);
     g2 = Solver2D();
 }
 
@@ -85,6 +88,7 @@ void fteik_solver2d_initialize(int nz, int nx, double z0, double x0,
     *ierr = 0;
     if (verbose > 0) printf("fteik_solver2d_initialize64f: Initializing...\n");
     free2d();
+    model_free();
     g2.verbose = verbose;
     g2.convTol = convTol;
     if (nz < 3 || nx < 3 || dz <= 0.0 || dx <= 0.0) {
@@ -93,7 +97,6 @@ void fteik_solver2d_initialize(int nz, int nx, double z0, double x0,
         return;
     }
     g2.nz = nz; g2.nx = nx; g2.z0 = z0; g2.x0 = x0; g2.dz = dz; g2.dx = dx;
-    model_set(false, nz, nx, 1);
     fteik_solver2d_setNumberOfSweeps(nsweep, ierr);
     if (*ierr) {
         msg("fteik_solver2d_initialize64f: Failed to set max number of sweeps");
@@ -114,6 +117,7 @@ void fteik_solver2d_initialize(int nz, int nx, double z0, double x0,
         *ierr = 1;
         return;
     }
+    model_set(false, nz, nx, 1);
     g2.init = true;
 }
 
@@ -209,6 +213,8 @@ void fteik_solver2d_setReceivers64f(int nrec, const double *zrec,
         return;
     }
     g2.nrec = nrec;
+    g2.zr.assign(zrec, zrec + nrec);
+    g2.xr.assign(xrec, xrec + nrec);
     size_rec2d();
     if (e) {
         msg("fteik_solver2d_setReceivers64f: Failed to set receivers");
@@ -287,6 +293,7 @@ void fteik_solver2d_solveSourceLSM(int isrc, int *ierr)
     static const char *who = "fteik_solver2d_solveSourceLSM";
     *ierr = 0;
     g2.haveTimes = false;
+    g2.haveGrad = false;
     g2.srcNumber = -1;
     if (!g2.init || !g2.haveModel) {
         msg("%s: Model not yet set", who);
@@ -312,6 +319,7 @@ void fteik_solver2d_solveSourceFSM(int isrc, int *ierr)
     static const char *who = "fteik_solver2d_solveSourceFSM";
     *ierr = 0;
     g2.haveTimes = false;
+    g2.haveGrad = false;
     if (!g2.init || !g2.haveModel) { msg("%s: Model not yet set", who);
                                      *ierr = 1; return; }
     const int nsrc = (int) g2.src.size();
@@ -346,6 +354,7 @@ void fteik_solver2d_solveLSM(int *ierr)
     const int chunk = (int) std::max(1L, std::min<long>(nsrc,
                                 (1L << 28)/std::max(1L, ngrd)));
     g2.haveTimes = false;
+    g2.haveGrad = false;
     for (int f = 0; f < nsrc; f += chunk) {
         int n = std::min(chunk, nsrc - f);
         if (!solve2d_range(f, n, who)) {
@@ -440,20 +449,86 @@ void fteik_solver2d_getTravelTimes64f(int ldr, double *ttr, int *ierr)
                (size_t) g2.nrec*sizeof(double));
 }
 
-void fteik_solver2d_traceRaysToReceivers(int *ierr)
+// solver2D.F90:984-1032 on the device, once per solved field.
+static bool gradient2d(const char *who)
 {
-    msg("fteik_solver2d_traceRaysToReceivers: ray tracing is not part of "
-        "this library");
-    *ierr = 1;
+    if (g2.haveGrad) return true;
+    const long ngrd = (long) g2.nz*g2.nx;
+    if ((!g2.d_gx && !dalloc(&g2.d_gx, (size_t) ngrd)) ||
+        (!g2.d_gz && !dalloc(&g2.d_gz, (size_t) ngrd))) {
+        msg("%s: out of GPU memory (gradient)", who);
+        return false;
+    }
+    if (fteik_k_fd_gradient2d(g2.d_gx, g2.d_gz, g2.d_tt, g2.nz, g2.nx, g2.dz,
+                              g2.dx, g_dev.stream) != 0 || !sync_ok()) {
+        msg("%s: GPU error (gradient)", who);
+        return false;
+    }
+    g2.haveGrad = true;
+    return true;
 }
 
+// solver2D.F90:310-355: rays from the points to the solved source (the
+// field of the last source solved by solveSourceLSM or solveLSM).
 void fteik_solver2d_traceRaysToPoints(int np, const double *xp,
                                       const double *zp, int *ierr)
 {
-    (void) np; (void) xp; (void) zp;
-    msg("fteik_solver2d_traceRaysToPoints: ray tracing is not part of this "
-        "library");
-    *ierr = 1;
+    static const char *who = "fteik_solver2d_traceRaysToPoints";
+    *ierr = 0;
+    if (!g2.haveTimes || g2.srcNumber < 1) {
+        msg("%s: Travel time field not yet computed", who);
+        *ierr = 1;
+        return;
+    }
+    if (!gradient2d(who)) { *ierr = 1; return; }
+    RayGeom G;
+    G.is3d = false; G.nz = g2.nz; G.nx = g2.nx; G.ny = 1;
+    G.dz = g2.dz; G.dx = g2.dx; G.dy = 1.0;
+    G.z0 = g2.z0; G.x0 = g2.x0; G.y0 = 0.0;
+    const SrcInfo &s = g2.src[(size_t) (g2.srcNumber - 1)];
+    if (!trace_rays(G, g2.d_slow, nullptr, g2.d_gx, g2.d_gz, s.zsi, s.xsi, 1,
+                    np, xp, nullptr, zp, who))
+        *ierr = 1;
+}
+
+// solver2D.F90:283-302
+void fteik_solver2d_traceRaysToReceivers(int *ierr)
+{
+    *ierr = 0;
+    if (g2.nrec < 1) {
+        msg("fteik_solver2d_traceRaysToReceivers: No receivers set");
+        *ierr = 1;
+        return;
+    }
+    fteik_solver2d_traceRaysToPoints(g2.nrec, g2.xr.data(), g2.zr.data(),
+                                     ierr);
+    if (*ierr != 0) {
+        msg("fteik_solver2d_traceRaysToReceivers: Failed to compute rays");
+        *ierr = 1;
+    }
+}
+
+// The gradient of the solved field (solver2D.F90:984-1032), per node in
+// natural order (z fastest).
+void fteik_solver2d_getTravelTimeGradient64f(int ngin, double *tgradx,
+                                             double *tgradz, int *ierr)
+{
+    static const char *who = "fteik_solver2d_getTravelTimeGradient64f";
+    *ierr = 0;
+    if (!g2.haveTimes) {
+        msg("%s: Travel times not yet computed", who);
+        *ierr = 1;
+        return;
+    }
+    const long ngrd = (long) g2.nz*g2.nx;
+    if (ngin != ngrd) {
+        msg("%s: ngin=%d /= ngrd =%ld", who, ngin, ngrd);
+        *ierr = 1;
+        return;
+    }
+    if (!gradient2d(who) ||
+        !d2h(tgradx, g2.d_gx, (size_t) ngrd*sizeof(double)) ||
+        !d2h(tgradz, g2.d_gz, (size_t) ngrd*sizeof(double))) *ierr = 1;
 }
 
 } // extern "C"

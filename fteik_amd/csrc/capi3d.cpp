@@ -22,8 +22,11 @@ struct Solver3D {
     std::vector<SrcInfo> src;
     std::vector<int> ri;
     std::vector<double> rd;
+    std::vector<double> zr, xr, yr;      // receiver positions (model coords)
     int nrec = 0;
     bool haveRec = false;
+    double *d_grad = nullptr;            // [x y z] per node, on demand
+    bool haveGrad = false;
     double *d_slow = nullptr, *d_tt = nullptr, *d_scr = nullptr;
     float *d_slow32 = nullptr, *d_tt32 = nullptr;
     void *d_cst = nullptr, *d_cst32 = nullptr;
@@ -63,7 +66,7 @@ void free3d()
     dfree(g3.d_mb); dfree(g3.d_mbx); dfree(g3.d_mby);
     dfree(g3.d_colmod); dfree(g3.d_lastproc); dfree(g3.d_plan);
     dfree(g3.d_prog);
-    dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr);
+    dfree(g3.d_ri); dfree(g3.d_rd); dfree(g3.d_ttr); dfree(g3.d_grad);
     if (g3.ev0) hipEventDestroy(g3.ev0);
     if (g3.ev1) hipEventDestroy(g3.ev1);
     g3 = Solver3D();
@@ -245,6 +248,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     *ierr = 0;
     if (verbose > 0) printf("fteik_solver3d_initialize: Initializing...\n");
     free3d();
+    model_free();
     g3.verbose = verbose;
     g3.convTol = convTol;
     if (nz < 3 || nx < 3 || ny < 3) {
@@ -262,7 +266,6 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
         return;
     }
     g3.nz = nz; g3.nx = nx; g3.ny = ny;
-    model_set(true, nz, nx, ny);
     g3.z0 = z0; g3.x0 = x0; g3.y0 = y0;
     g3.dz = dz; g3.dx = dx; g3.dy = dy;
     fteik_solver3d_setNumberOfSweeps(nsweep, ierr);
@@ -355,6 +358,7 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
     if (!g3.cooperative) g3.schedule = 1;
     if (!sync_ok()) { msg("fteik_solver3d_initialize: GPU error"); *ierr = 1;
                       return; }
+    model_set(true, nz, nx, ny);
     g3.init = true;
 }
 
@@ -494,6 +498,9 @@ void fteik_solver3d_setReceivers64f(int nrec, const double *zrec,
     }
     g3.nrec = nrec;
     g3.haveRec = true;
+    g3.zr.assign(zrec, zrec + nrec);
+    g3.xr.assign(xrec, xrec + nrec);
+    g3.yr.assign(yrec, yrec + nrec);
     if (e) {
         msg("fteik_solver3d_setReceivers64f: Failed to set receivers");
         *ierr = e;
@@ -578,6 +585,59 @@ void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr)
     *ntiles = (int) g3.tileOK.size();
 }
 
+// Times a complete solve of source isrc with every tile shape usable on
+// this grid (in the current precision mode, skipping and overlap settings)
+// and keeps the fastest: the FP64 sweeps' shape in FP64 mode, both shapes
+// in the mixed mode.  Every shape gives the same field bit for bit (any
+// order that respects the levels rounds alike), so the solver is left
+// holding the solved field of isrc.
+void fteik_solver3d_autotune(int isrc, int *tile, int *tile32, int *ierr)
+{
+    static const char *who = "fteik_solver3d_autotune";
+    *ierr = 0;
+    *tile = g3.tile;
+    *tile32 = g3.tile32;
+    if (!g3.init || !g3.haveModel || isrc < 1 || isrc > (int) g3.src.size()) {
+        msg("%s: solver, model or source %d not set", who, isrc);
+        *ierr = 1;
+        return;
+    }
+    if (g3.schedule != 0) {
+        msg("%s: the per-level schedule has no tile shape to tune", who);
+        *ierr = 1;
+        return;
+    }
+    const int t0 = g3.tile, t032 = g3.tile32;
+    const bool mixed = g3.precision == 1;
+    int best = -1;
+    double bestMs = 0.0;
+    for (int t = 0; t < (int) g3.tileOK.size(); t++) {
+        if (!g3.tileOK[(size_t) t]) continue;
+        g3.tile = t;
+        g3.tile32 = mixed ? t : t032;
+        int e = 0;
+        fteik_solver3d_solveSourceLSM(isrc, &e);
+        if (e == 1) continue;                  // shape failed: not a winner
+        if (g3.verbose > 1)
+            printf("%s: tile %d: %.3f ms\n", who, t, g3.last_ms);
+        if (best < 0 || g3.last_ms < bestMs) { best = t; bestMs = g3.last_ms; }
+    }
+    if (best < 0) {
+        g3.tile = t0;
+        g3.tile32 = t032;
+        msg("%s: no tile shape solved source %d", who, isrc);
+        *ierr = 1;
+        return;
+    }
+    g3.tile = best;
+    g3.tile32 = mixed ? best : t032;
+    int e = 0;
+    fteik_solver3d_solveSourceLSM(isrc, &e);    // leave the winner's solve
+    if (e == 1) { *ierr = 1; return; }
+    *tile = g3.tile;
+    *tile32 = g3.tile32;
+}
+
 void fteik_solver3d_setTileSkipping(int on, int *ierr)
 {
     *ierr = 0;
@@ -646,6 +706,7 @@ void fteik_solver3d_solveSourceLSM(int isrc, int *ierr)
     static const char *who = "fteik_solver3d_solveSourceLSM";
     *ierr = 0;
     g3.haveTimes = false;
+    g3.haveGrad = false;
     g3.srcNumber = -1;
     if (!g3.init || !g3.haveModel) {
         msg("%s: Model not yet set", who);
@@ -866,21 +927,87 @@ void fteik_solver3d_getTravelTimes64f(int nrec, double *ttr, int *ierr)
         !d2h(ttr, g3.d_ttr, (size_t) n*sizeof(double))) *ierr = 1;
 }
 
-void fteik_solver3d_traceRaysToReceivers(int *ierr)
+// solver3D.F90:1249-1321 on the device, once per solved field.
+static bool gradient3d(const char *who)
 {
-    msg("fteik_solver3d_traceRaysToReceivers: ray tracing is not part of "
-        "this library");
-    *ierr = 1;
+    if (g3.haveGrad) return true;
+    const long ngrd = (long) g3.nz*g3.nx*g3.ny;
+    if (!g3.d_grad && !dalloc(&g3.d_grad, 3*(size_t) ngrd)) {
+        msg("%s: out of GPU memory (gradient)", who);
+        return false;
+    }
+    if (fteik_k_fd_gradient3d(g3.d_grad, g3.d_tt, g3.nz, g3.nx, g3.ny, g3.dz,
+                              g3.dx, g3.dy, g_dev.stream) != 0 || !sync_ok()) {
+        msg("%s: GPU error (gradient)", who);
+        return false;
+    }
+    g3.haveGrad = true;
+    return true;
 }
 
+// solver3D.F90:784-822: rays from the points to the solved source, down
+// the finite-difference gradient of the field (the reference computes the
+// gradient and sets the destinations; its 3D tracer, rays.F90:442-447, is
+// an empty stub, so the tracing itself is this library's).
 void fteik_solver3d_traceRaysToPoints(int np, const double *zp,
                                       const double *xp, const double *yp,
                                       int *ierr)
 {
-    (void) np; (void) zp; (void) xp; (void) yp;
-    msg("fteik_solver3d_traceRaysToPoints: ray tracing is not part of this "
-        "library");
-    *ierr = 1;
+    static const char *who = "fteik_solver3d_traceRaysToPoints";
+    *ierr = 0;
+    if (!g3.haveTimes || g3.srcNumber < 1) {
+        msg("%s: Travel time field not yet computed", who);
+        *ierr = 1;
+        return;
+    }
+    if (!gradient3d(who)) { *ierr = 1; return; }
+    RayGeom G;
+    G.is3d = true; G.nz = g3.nz; G.nx = g3.nx; G.ny = g3.ny;
+    G.dz = g3.dz; G.dx = g3.dx; G.dy = g3.dy;
+    G.z0 = g3.z0; G.x0 = g3.x0; G.y0 = g3.y0;
+    const SrcInfo &s = g3.src[(size_t) (g3.srcNumber - 1)];
+    if (!trace_rays(G, g3.d_slow, g3.d_grad, nullptr, nullptr, s.zsi, s.xsi,
+                    s.ysi, np, xp, yp, zp, who))
+        *ierr = 1;
+}
+
+// solver3D.F90:756-777
+void fteik_solver3d_traceRaysToReceivers(int *ierr)
+{
+    *ierr = 0;
+    if (!g3.haveRec || g3.nrec < 1) {
+        msg("fteik_solver3d_traceRaysToReceivers: No receivers set");
+        *ierr = 1;
+        return;
+    }
+    fteik_solver3d_traceRaysToPoints(g3.nrec, g3.zr.data(), g3.xr.data(),
+                                     g3.yr.data(), ierr);
+    if (*ierr != 0) {
+        msg("fteik_solver3d_traceRaysToReceivers: Failed to compute rays");
+        *ierr = 1;
+    }
+}
+
+// The gradient of the solved field (solver3D.F90:1249-1321): [x y z] per
+// node, natural node order, 3*ngrd values.
+void fteik_solver3d_getTravelTimeGradient64f(int ngin, double *tgrad,
+                                             int *ierr)
+{
+    static const char *who = "fteik_solver3d_getTravelTimeGradient64f";
+    *ierr = 0;
+    if (!g3.haveTimes) {
+        msg("%s: Travel times not yet computed", who);
+        *ierr = 1;
+        return;
+    }
+    const long ngrd = (long) g3.nz*g3.nx*g3.ny;
+    if (ngin != ngrd) {
+        msg("%s: ngin=%d /= ngrd =%ld", who, ngin, ngrd);
+        *ierr = 1;
+        return;
+    }
+    if (!gradient3d(who) ||
+        !d2h(tgrad, g3.d_grad, 3*(size_t) ngrd*sizeof(double))) *ierr = 1;
 }
 
 void fteik_amd_evalLocalSolver3d(int n, const double *in, const int *node,

@@ -476,6 +476,38 @@ void fteik_batch3d_solve(int nsweep, double eps, int *ierr)
     }
 }
 
+// Times fteik_batch3d_solve(nsweep, eps) with every tile shape the batch
+// can use (flag and mailbox tiles) and keeps the fastest; the batch is left
+// solved with it.  The fields do not depend on the shape.
+void fteik_batch3d_autotune(int nsweep, double eps, int *tile, int *ierr)
+{
+    static const char *who = "fteik_batch3d_autotune";
+    *ierr = 0;
+    *tile = gb.mbTile >= 0 ? gb.mbTile : gb.tile;
+    if (!gb.init) { msg("%s: batch not initialized", who); *ierr = 1;
+                    return; }
+    const int keepTile = gb.tile, keepMb = gb.mbTile;
+    int best = -1;
+    double bestMs = 0.0;
+    for (int t = 0; t < fteik_k_num_tiles(); t++) {
+        int e = 0;
+        fteik_batch3d_setTile(t, &e);
+        if (e) continue;
+        fteik_batch3d_solve(nsweep, eps, &e);
+        if (e == 1) continue;
+        if (best < 0 || gb.last_ms < bestMs) { best = t; bestMs = gb.last_ms; }
+    }
+    gb.tile = keepTile;
+    gb.mbTile = keepMb;
+    if (best < 0) { msg("%s: no tile shape solved the batch", who);
+                    *ierr = 1; return; }
+    int e = 0;
+    fteik_batch3d_setTile(best, &e);
+    fteik_batch3d_solve(nsweep, eps, &e);
+    if (e == 1) { *ierr = 1; return; }
+    *tile = best;
+}
+
 void fteik_batch3d_getLastSolveInfo(double *ms, int *corrIterations,
                                     double *updates, int *ierr)
 {

@@ -113,4 +113,34 @@ float weight_norm(const float *w, int lntf);
 void mask_skipped(float *o, float *w, float *w2, const unsigned char *sk,
                   size_t n);
 
+// ------------------------------------------------------------------- rays
+// Geometry of a ray trace (2D: is3d false, ny = 1).
+struct RayGeom {
+    bool is3d = true;
+    int nz = 0, nx = 0, ny = 1;
+    double dz = 0, dx = 0, dy = 1, z0 = 0, x0 = 0, y0 = 0;
+};
+
+// The last trace's rays (the reference's FTEIK_RAYS64F state).
+struct RayStore {
+    int integrator = 1;              // 1 Euler, 2 midpoint, 3 RK4
+    int nrays = 0, maxpts = 0;
+    bool is3d = true;
+    double x0 = 0, y0 = 0, z0 = 0;
+    std::vector<double> pts;         // [nrays][maxpts][3], grid-relative
+    std::vector<int> npts, status;
+    std::vector<double> time;
+};
+extern RayStore gr;
+
+// Traces rays from the np points (model coordinates) to the source node of
+// cell (zsi, xsi, ysi) through the device gradient (d_g3 in 3D, d_gx and
+// d_gz in 2D) and slowness; false (with messages naming `who`) if a
+// destination lies outside the model or a ray does not reach the source.
+bool trace_rays(const RayGeom &G, const double *d_slow, const double *d_g3,
+                const double *d_gx, const double *d_gz, int zsi, int xsi,
+                int ysi, int np, const double *x, const double *y,
+                const double *z, const char *who);
+void rays_clear();
+
 } // namespace fteik_capi

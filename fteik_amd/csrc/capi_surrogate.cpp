@@ -37,6 +37,10 @@ struct Surrogate {
     float *d_pos = nullptr, *d_mis = nullptr, *d_zp = nullptr,
           *d_yall = nullptr;
     unsigned short *d_ha = nullptr, *d_hb = nullptr;
+    // proposals from the network (default) or from the straight-ray
+    // baseline alone (network output taken as zero: the ablation)
+    bool useNet = true;
+    float *d_zero = nullptr;           // ntf zero biases for the baseline
 };
 Surrogate gs;
 
@@ -80,6 +84,7 @@ void free_surrogate()
     dfree(gs.d_geo); dfree(gs.d_p); dfree(gs.d_g); dfree(gs.d_m); dfree(gs.d_v);
     dfree(gs.d_pb); dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y);
     dfree(gs.d_dh); dfree(gs.d_dy); dfree(gs.d_dz); dfree(gs.d_loss);
+    dfree(gs.d_zero);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     free_inv_work();
     gs = Surrogate();
@@ -221,11 +226,13 @@ void fteik_surrogate_initialize(int ntf, int nz, int nx, int ny, double dz,
               dalloc(&gs.d_p, (size_t) o) && dalloc(&gs.d_g, (size_t) o) &&
               dalloc(&gs.d_m, (size_t) o) && dalloc(&gs.d_v, (size_t) o) &&
               dalloc(&gs.d_pb, (size_t) o) && dalloc(&gs.d_loss, 1) &&
+              dalloc(&gs.d_zero, 64) &&
               h2d(gs.d_geo, &g, sizeof(g)) &&
               h2d(gs.d_p, p.data(), (size_t) o*sizeof(float));
     if (ok) {
         hipMemsetAsync(gs.d_m, 0, (size_t) o*sizeof(float), g_dev.stream);
         hipMemsetAsync(gs.d_v, 0, (size_t) o*sizeof(float), g_dev.stream);
+        hipMemsetAsync(gs.d_zero, 0, 64*sizeof(float), g_dev.stream);
         ok = fteik_k_f2bf(gs.d_p, gs.d_pb, o, g_dev.stream) == 0 && sync_ok();
     }
     if (!ok) { msg("%s: GPU error", who); free_surrogate(); *ierr = 1; return; }
@@ -331,6 +338,20 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
     *lossOut = tail/tailN;
 }
 
+// Proposals from the network (on = 1, default) or from the straight-ray
+// baseline |x - station| / v0 alone (on = 0), with the same Gauss-Newton
+// steps, cube scoring and full-search fallback: what the network adds.
+void fteik_surrogate_setNetworkProposals(int on, int *ierr)
+{
+    *ierr = 0;
+    if (on != 0 && on != 1) {
+        msg("fteik_surrogate_setNetworkProposals: on must be 0 or 1");
+        *ierr = 1;
+        return;
+    }
+    gs.useNet = on == 1;
+}
+
 // Surrogate-pruned location of events evnmbr[0..n-1] (fresh origin-time
 // state): damped Gauss-Newton on the network's misfit from `starts`
 // starting points, exact scoring of the 4 x 4 x 4 cells around the best,
@@ -385,22 +406,31 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
               h2d(d_xn, xn.data(), xn.size()*sizeof(float)) &&
               h2d(d_sk, sk.data(), sk.size());
     ok = ok && fteik_k_inv_start(d_pos, R, starts, s) == 0;
+    if (ok && !gs.useNet)
+        // baseline proposals: network outputs and tangents all zero, so the
+        // Gauss-Newton steps fit the straight-ray times alone
+        ok = fteik_k_fill_u64((unsigned long long *) d_yall, 2*R*T, 0ull,
+                              s) == 0;
     for (int it = 0; it <= iterations && ok; it++) {
-        // forward with tangents over 4R rows
-        ok = fteik_k_inv_layer1(d_pos, R, gs.d_p + gs.oW1, gs.d_p + gs.ob1,
-                                d_zp, d_ha, H, s) == 0;
-        unsigned short *hin = d_ha, *hout = d_hb;
-        for (int l = 0; l < 3 && ok; l++) {
-            // fused GEMM + activation (the GEMM then k_inv_act, bitwise)
-            ok = fteik_k_inv_hidden(hin, R, gs.d_pb + gs.oW[l],
-                                    gs.d_p + gs.ob[l], hout, H, s) == 0;
-            std::swap(hin, hout);
+        if (gs.useNet) {
+            // forward with tangents over 4R rows
+            ok = fteik_k_inv_layer1(d_pos, R, gs.d_p + gs.oW1,
+                                    gs.d_p + gs.ob1, d_zp, d_ha, H, s) == 0;
+            unsigned short *hin = d_ha, *hout = d_hb;
+            for (int l = 0; l < 3 && ok; l++) {
+                // fused GEMM + activation (the GEMM then k_inv_act, bitwise)
+                ok = fteik_k_inv_hidden(hin, R, gs.d_pb + gs.oW[l],
+                                        gs.d_p + gs.ob[l], hout, H, s) == 0;
+                std::swap(hin, hout);
+            }
+            ok = ok && fteik_k_gemm_bf16(hin, H, 1, gs.d_pb + gs.oW5, 1, H,
+                                         d_yall, T, (int) (4*R), T, H,
+                                         nullptr, 0, s) == 0;
         }
-        ok = ok && fteik_k_gemm_bf16(hin, H, 1, gs.d_pb + gs.oW5, 1, H, d_yall,
-                                     T, (int) (4*R), T, H, nullptr, 0, s) == 0
-                && fteik_k_inv_gn(d_yall, R, T, gs.d_p + gs.ob5, gs.d_geo, d_o,
-                                  d_w, gl.lntf, starts, d_pos, d_mis,
-                                  it < iterations ? 1 : 0, s) == 0;
+        ok = ok && fteik_k_inv_gn(d_yall, R, T,
+                                  gs.useNet ? gs.d_p + gs.ob5 : gs.d_zero,
+                                  gs.d_geo, d_o, d_w, gl.lntf, starts, d_pos,
+                                  d_mis, it < iterations ? 1 : 0, s) == 0;
     }
     ok = ok && fteik_k_inv_score(d_pos, d_mis, starts, n, gs.d_geo, gl.d_tab,
                                  gl.ldgrd, d_o, d_w, d_sk, d_xn, gl.lntf,

@@ -57,9 +57,27 @@ struct BatchGrid {
     int nz, nx, ny, zsi, xsi, ysi;
 };
 
+// Grid of a ray trace (rays.hip): 2D grids have ny = 1 and is3d = 0.
+struct RayGrid {
+    int is3d, nz, nx, ny;
+    double dz, dx, dy;
+};
+
 } // namespace fteik
 
 extern "C" {
+// rays.hip
+int fteik_k_fd_gradient3d(double* g, const double* tt, int nz, int nx,
+                          int ny, double dz, double dx, double dy,
+                          hipStream_t stream);
+int fteik_k_fd_gradient2d(double* gx, double* gz, const double* tt, int nz,
+                          int nx, double dz, double dx, hipStream_t stream);
+int fteik_k_trace_rays(const fteik::RayGrid* rg, const double* slow,
+                       const double* g3, const double* gx, const double* gz,
+                       int integrator, double xs, double ys, double zs,
+                       const double* start, int nray, int maxpts,
+                       double* pts, int* npts, double* time, int* status,
+                       hipStream_t stream);
 // batch3d.hip
 int fteik_k_batch_map(int tile, int ngrids, const int* dims, int* map,
                       int map_cap, int* nflags);

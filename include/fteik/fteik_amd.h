@@ -78,12 +78,24 @@ void fteik_solver3d_setSphereToCartEpsilon(double eps, int *ierr);
 void fteik_solver3d_setVerbosity(int verbose);
 /* solver3D.F90:206 */
 void fteik_solver3d_setConvergenceTolerance(double convTol);
-/* solver3D.F90:756, :784 -- ray tracing is not part of this library:
- * these return ierr = 1. */
+/* solver3D.F90:756, :784: rays from the receivers / the points (model
+ * coordinates) to the solved source, down the finite-difference gradient
+ * of the field (dx/ds = -grad T / u, integrator of
+ * fteik_rays_setODEIntegrator).  Paths and times are read with the
+ * fteik_rays_get* functions.  The reference computes the gradient and
+ * checks the destinations but its 3D tracer is an empty stub
+ * (rays.F90:442-447); this library traces the rays on the GPU.  ierr = 1
+ * if the field is not solved, a point lies outside the model or a ray does
+ * not reach the source. */
 void fteik_solver3d_traceRaysToReceivers(int *ierr);
 void fteik_solver3d_traceRaysToPoints(int np, const double *zp,
                                       const double *xp, const double *yp,
                                       int *ierr);
+/* EXTENSION: the finite-difference gradient of the solved field
+ * (solver3D.F90:1249-1321: forward differences, backward on the last
+ * face), [dT/dx dT/dy dT/dz] per node in natural order: tgrad[3*ngrd]. */
+void fteik_solver3d_getTravelTimeGradient64f(int ngrd, double *tgrad,
+                                             int *ierr);
 /* EXTENSION: 0 = FP64 sweeps (reference semantics, default);
  * 1 = mixed: FP64 initialization sweeps, nsweep FP32 Gauss-Seidel
  *     iterations, promotion to FP64 with a small upward margin, then FP64
@@ -113,6 +125,12 @@ void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
  * fails the solve with ierr = 1 rather than hanging. */
 void fteik_solver3d_setTile(int tile, int tile32, int *ierr);
 void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
+/* EXTENSION: the autotuner.  Solves source isrc (1-based) once with every
+ * tile shape usable on this grid, in the current precision, tile-skipping
+ * and overlap settings, keeps the fastest (the FP64 shape in FP64 mode,
+ * both shapes in the mixed mode) and returns it.  All shapes give the same
+ * field bit for bit; the solver is left holding the solved field of isrc. */
+void fteik_solver3d_autotune(int isrc, int *tile, int *tile32, int *ierr);
 /* EXTENSION: skip, in the Gauss-Seidel iterations, every tile of the
  * pipelined sweep whose columns have not changed since its last pass in the
  * same direction and whose upwind tiles are skipped too: such a pass would
@@ -178,10 +196,30 @@ void fteik_solver2d_setNumberOfSweeps(int nsweep, int *ierr);
 void fteik_solver2d_setSphereToCartEpsilon(double eps, int *ierr);
 void fteik_solver2d_setVerbosity(int verbose);
 void fteik_solver2d_setConvergenceTolerance(double convTol);
-/* solver2D.F90:283, :310 -- not part of this library: ierr = 1. */
+/* solver2D.F90:283, :310: rays to the last solved source (as the 3D
+ * functions; the reference's 2D tracer, rays.F90:449-669, stops after its
+ * first ray). */
 void fteik_solver2d_traceRaysToReceivers(int *ierr);
 void fteik_solver2d_traceRaysToPoints(int np, const double *xp,
                                       const double *zp, int *ierr);
+/* EXTENSION: the finite-difference gradient of the solved 2D field
+ * (solver2D.F90:984-1032), dT/dx and dT/dz per node in natural order. */
+void fteik_solver2d_getTravelTimeGradient64f(int ngrd, double *tgradx,
+                                             double *tgradz, int *ierr);
+
+/* ================================ Rays =============================== */
+/* rays.F90:211: 1 = Euler (default), 2 = midpoint, 3 = classical RK4; an
+ * invalid value keeps Euler and returns ierr = 1. */
+void fteik_rays_setODEIntegrator(int integrator, int *ierr);
+/* EXTENSION: the last trace's rays (the reference keeps them in its
+ * module, rays.F90:23, without getters).  getRayPath: npts points
+ * (x, y, z; model coordinates, y = 0 for 2D) from the receiver to the
+ * source into xyz[3*nwork], and the ray's time (seconds: path length times
+ * cell slowness, summed over the steps). */
+void fteik_rays_getNumberOfRays(int *nrays, int *ierr);
+void fteik_rays_getTravelTimes64f(int nrays, double *t, int *ierr);
+void fteik_rays_getRayPath(int iray, int nwork, double *xyz, int *npts,
+                           double *time, int *ierr);
 
 /* ============================== Locator ============================== */
 /* locate.F90:142 */
@@ -227,7 +265,6 @@ void locate_setReferenceCompatibility(int mode, int *ierr);
  * tables, each with a fresh origin-time estimate. */
 void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
                          double *t0Opt, double *objOpt, int *ierr);
-
 /* ========================= Batched 3D solver ========================= */
 /* EXTENSION (no counterpart in the reference, which solves one source at a
  * time, solver3D.F90:1156): a batch of ngrids independent single-source
@@ -250,6 +287,10 @@ void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
 /* tile shape of the batch (see fteik_solver3d_setTile), flag (mode 0) or
  * mailbox (modes 1, 2); it serves every sweep (default: mailbox tile 12) */
 void fteik_batch3d_setTile(int tile, int *ierr);
+/* EXTENSION: the batch autotuner: times fteik_batch3d_solve(nsweep, eps)
+ * with every tile shape (flag and mailbox tiles), keeps the fastest and
+ * returns it; the batch is left solved with it. */
+void fteik_batch3d_autotune(int nsweep, double eps, int *tile, int *ierr);
 void fteik_batch3d_solve(int nsweep, double eps, int *ierr);
 void fteik_batch3d_getLastSolveInfo(double *ms, int *corrIterations,
                                     double *updates, int *ierr);
@@ -291,6 +332,11 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
 void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
                                   int iterations, int *optIndx, double *t0Opt,
                                   double *objOpt, int *nFallback, int *ierr);
+/* EXTENSION: on = 1 (default): proposals from the network; on = 0: from
+ * the straight-ray baseline |x - station| / v0 alone (the network's output
+ * taken as zero), with the same Gauss-Newton steps, cube scoring and
+ * full-search fallback -- the method without the learned part. */
+void fteik_surrogate_setNetworkProposals(int on, int *ierr);
 /* diagnostic: C = A B (row-major FP32 in/out) through the BF16 MFMA GEMM */
 void fteik_amd_gemmBf16(int M, int N, int K, const float *A, const float *B,
                         float *C, int *ierr);

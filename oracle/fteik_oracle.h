@@ -143,6 +143,22 @@ int or_locate_event(int ntf, int ngrd, int ldgrd, const float *tables,
                     float *objwork, int nthreads, double *t0opt,
                     double *objopt);
 
+/* ---------------------------------------- gradients and ray tracing
+ * (fteik_oracle_rays.c) */
+void or_fd_gradient3d(int nz, int nx, int ny, double dz, double dx,
+                      double dy, const double *tt, double *tgrad);
+void or_fd_gradient2d(int nz, int nx, double dz, double dx, const double *tt,
+                      double *tgradx, double *tgradz);
+int or_trace_ray3d(int nz, int nx, int ny, double dz, double dx, double dy,
+                   const double *slow, const double *tgrad, int integrator,
+                   double xs, double ys, double zs, double xr, double yr,
+                   double zr, int maxpts, double *pts, int *npts,
+                   double *time);
+int or_trace_ray2d(int nz, int nx, double dz, double dx, const double *slow,
+                   const double *tgradx, const double *tgradz, int integrator,
+                   double xs, double zs, double xr, double zr, int maxpts,
+                   double *pts, int *npts, double *time);
+
 #ifdef __cplusplus
 }
 #endif

@@ -330,6 +330,55 @@ class Locator:
         return opt, t0[0], ob[0]
 
 
+# ------------------------------------------------- gradients and rays
+def fd_gradient3d(tt, nz, nx, ny, dz, dx, dy):
+    """solver3D.F90:1249-1321: [x y z] per node, natural node order."""
+    t = np.ascontiguousarray(tt, dtype=np.float64)
+    g = np.zeros(3 * nz * nx * ny)
+    lib().or_fd_gradient3d(nz, nx, ny, c_double(dz), c_double(dx),
+                           c_double(dy), _d(t), _d(g))
+    return g
+
+
+def fd_gradient2d(tt, nz, nx, dz, dx):
+    """solver2D.F90:984-1032: (d/dx, d/dz) per node."""
+    t = np.ascontiguousarray(tt, dtype=np.float64)
+    gx = np.zeros(nz * nx)
+    gz = np.zeros(nz * nx)
+    lib().or_fd_gradient2d(nz, nx, c_double(dz), c_double(dx), _d(t), _d(gx),
+                           _d(gz))
+    return gx, gz
+
+
+def trace_ray3d(slow, tgrad, nz, nx, ny, dz, dx, dy, integrator, src, rec,
+                maxpts):
+    """One ray from rec (x, y, z) to the source node src (x, y, z), both
+    grid-relative metres.  Returns (ok, points [n x 3], time)."""
+    pts = np.zeros(3 * maxpts)
+    n = ctypes.c_int()
+    t = np.zeros(1)
+    rc = lib().or_trace_ray3d(nz, nx, ny, c_double(dz), c_double(dx),
+                              c_double(dy), _d(slow), _d(tgrad), integrator,
+                              c_double(src[0]), c_double(src[1]),
+                              c_double(src[2]), c_double(rec[0]),
+                              c_double(rec[1]), c_double(rec[2]), maxpts,
+                              _d(pts), ctypes.byref(n), _d(t))
+    return rc == 0, pts[:3 * n.value].reshape(n.value, 3), t[0]
+
+
+def trace_ray2d(slow, gx, gz, nz, nx, dz, dx, integrator, src, rec, maxpts):
+    """2D analogue of trace_ray3d: src, rec are (x, z); points (x, 0, z)."""
+    pts = np.zeros(3 * maxpts)
+    n = ctypes.c_int()
+    t = np.zeros(1)
+    rc = lib().or_trace_ray2d(nz, nx, c_double(dz), c_double(dx), _d(slow),
+                              _d(gx), _d(gz), integrator, c_double(src[0]),
+                              c_double(src[1]), c_double(rec[0]),
+                              c_double(rec[1]), maxpts, _d(pts),
+                              ctypes.byref(n), _d(t))
+    return rc == 0, pts[:3 * n.value].reshape(n.value, 3), t[0]
+
+
 # ------------------------------------------------------ reference build
 _ref = None
 

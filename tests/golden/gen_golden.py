@@ -246,6 +246,29 @@ out["ls3d_init"] = np.array(rows_out_init)
 
 
 # --------------------------------------------------------------- 2D
+def solve2d_fsm_ref(nz, nx, dz, dx, vel, zs, xs, nsweep, eps):
+    """solver2D.F90:1042-1280 (the fast-sweeping-method driver)."""
+    R.fteik_solver2d_initialize(c_int(nz), c_int(nx), c_double(0.0),
+                                c_double(0.0), c_double(dz), c_double(dx),
+                                c_int(nsweep), c_double(eps), c_double(0.0),
+                                c_int(0), byref(ierr))
+    chk("solver2d_initialize (FSM)")
+    R.fteik_solver2d_setCellVelocityModel64f(c_int(vel.size), c_int(0),
+                                             D(vel), byref(ierr))
+    chk("solver2d vel (FSM)")
+    R.fteik_solver2d_setSources64f(c_int(1), D(np.array([zs])),
+                                   D(np.array([xs])), byref(ierr))
+    chk("solver2d src (FSM)")
+    R.fteik_solver2d_solveSourceFSM(c_int(1), byref(ierr))
+    chk("solver2d FSM solve")
+    tt = np.zeros(nz * nx)
+    R.fteik_solver2d_getTravelTimeField64f(c_int(tt.size), c_int(0), D(tt),
+                                           byref(ierr))
+    chk("solver2d FSM field")
+    R.fteik_solver2d_free()
+    return tt
+
+
 def solve2d_ref(nz, nx, dz, dx, vel, zs, xs, nsweep, eps, zr, xr):
     R.fteik_solver2d_initialize(c_int(nz), c_int(nx), c_double(0.0),
                                 c_double(0.0), c_double(dz), c_double(dx),
@@ -321,6 +344,11 @@ out["t2d_b_tt"] = tt3
 out["t2d_b_rec"] = np.stack([zr3, xr3])
 out["t2d_b_ttr"] = ttr3
 out["t2d_b_ttcell_xz"] = ttc3
+# the FSM driver on the same two configurations
+out["t2d_a_fsm_tt"] = solve2d_fsm_ref(nz2, nx2, dz2, dx2, vel2, zs2, xs2, 2,
+                                      5.0)
+out["t2d_b_fsm_tt"] = solve2d_fsm_ref(nz3, nx3, h3, h3, vel3, zs3, xs3, 3,
+                                      4.0)
 
 # 2D stencils on random states
 nz4, nx4 = 15, 12

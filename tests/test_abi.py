@@ -60,3 +60,20 @@ def test_product_has_no_cpu_fallback():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in src, f
+
+
+def test_index_functions_closed_form(lib):
+    """The pure index helpers (no GPU involved): model.F90:771-780,
+    :827-836 and the 0-based shim fwrapper.c:8-14."""
+    lib.fteik_model_grid2indexF.restype = ctypes.c_int
+    lib.fteik_model_velGrid2indexF.restype = ctypes.c_int
+    lib.fteik_model_grid2index.restype = ctypes.c_int
+    nz, nx = 34, 10
+    for (i, j, k) in ((1, 1, 1), (34, 10, 21), (7, 3, 11)):
+        g = lib.fteik_model_grid2indexF(i, j, k, nz, nz * nx)
+        assert g == (k - 1) * nz * nx + (j - 1) * nz + i
+        assert lib.fteik_model_grid2index(i - 1, j - 1, k - 1, nz,
+                                          nz * nx) == g - 1
+        c = lib.fteik_model_velGrid2indexF(i, j, k, nz - 1,
+                                           (nz - 1) * (nx - 1))
+        assert c == (k - 1) * (nz - 1) * (nx - 1) + (j - 1) * (nz - 1) + i

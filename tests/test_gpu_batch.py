@@ -186,3 +186,21 @@ def test_batch_mixed_precision_matches_fp64(tile):
               "max rel", float(rel.max()), flush=True)
         assert rel.max() <= 1e-6
     L.fteik_batch3d_free()
+
+
+def test_batch_autotune():
+    """fteik_batch3d_autotune picks a tile by timing the batch; the batch is
+    left solved and every grid equals the oracle."""
+    models = [hetero(nz, nx, ny, h, 80 + k)
+              for k, (nz, nx, ny, h, _) in enumerate(GRIDS)]
+    L = _batch(GRIDS, models, 2, 4.0)
+    t, ierr = c_int(-1), c_int(1)
+    L.fteik_batch3d_autotune(2, 4.0, byref(t), byref(ierr))
+    assert ierr.value == 0 and t.value >= 0
+    print("case: batch autotuned tile", t.value, flush=True)
+    for k, ((nz, nx, ny, h, (zc, xc, yc)), vel) in enumerate(zip(GRIDS,
+                                                                 models)):
+        ref, _, _ = oracle3d(vel, nz, nx, ny, h, zc * h, xc * h, yc * h, 2,
+                             4.0)
+        np.testing.assert_array_equal(_field(L, k, nz * nx * ny), ref)
+    L.fteik_batch3d_free()

@@ -101,10 +101,11 @@ def test_solve2d_matches_reference_fixtures(fixtures, case):
                                   fixtures["t2d_%s_ttr" % case])
     if case == "b":
         np.testing.assert_array_equal(g.cells(1), fixtures["t2d_b_ttcell_xz"])
-    # FSM entry point visits the same nodes and gives the same field
+    # the FSM entry point against the reference's own FSM driver
+    # (solver2D.F90:1042-1280, fixture from the reference build)
     g.L.fteik_solver2d_solveSourceFSM(1, byref(ierr))
     assert ierr.value == 0
-    np.testing.assert_array_equal(g.field(), fixtures["t2d_%s_tt" % case])
+    np.testing.assert_array_equal(g.field(), fixtures["t2d_%s_fsm_tt" % case])
 
 
 def test_solve2d_many_sources_one_launch():

@@ -142,6 +142,76 @@ def test_nodal_model_orders(order):
     np.testing.assert_array_equal(g.field(), ref)
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_cell_model_orders(order):
+    """Cell velocities in XYZ (order 1) and ZYX (order 2) order
+    (model.F90:469-559), the path python/fteik3d.py takes with ZYX."""
+    nz, nx, ny, h = 19, 16, 14, 25.0
+    vel_nat = hetero(nz, nx, ny, h, 50 + order)
+    v = vel_nat.reshape(ny - 1, nx - 1, nz - 1)
+    vo = (v.transpose(2, 0, 1) if order == 1 else v.transpose(1, 0, 2))
+    vo = np.ascontiguousarray(vo).ravel()
+    slow = orc.cell_slowness(vo, nz, nx, ny, order)
+    np.testing.assert_array_equal(slow, 1.0 / vel_nat)
+    zs, xs, ys = 6.6 * h, 3.3 * h, 9.9 * h
+    src = orc.source_locate(grid9(nz, nx, ny, h), zs, xs, ys, slow)
+    ref = orc.solve3d(slow, nz, nx, ny, h, h, h, src, 2, 5.0, schedule=1)
+    g = GPU3D(nz, nx, ny, h)
+    ierr = c_int(1)
+    g.L.fteik_solver3d_setCellVelocityModel64f(vo.size, order, _d(vo),
+                                               byref(ierr))
+    assert ierr.value == 0
+    g.set_sources(zs, xs, ys)
+    assert g.solve(1) == 0
+    np.testing.assert_array_equal(g.field(), ref)
+
+
+def test_index_helpers_round_trip():
+    """testing/tests.c:262-302: grid2indexF and index2gridF invert each
+    other on the 34 x 10 x 21 grid the solver was initialized with; the
+    0-based shim (fwrapper.c:8-14) and the cell index (model.F90:827-836)
+    by their closed forms; index2gridF follows the geometry of the solver
+    initialized last (ny = 1 for the 2D solver) and, with none, reports
+    ierr = 3."""
+    L = lib()
+    nz, nx, ny = 34, 10, 21
+    g = GPU3D(nz, nx, ny, 10.0)
+    i1, j1, k1, ierr = c_int(), c_int(), c_int(), c_int()
+    nzx = nz * nx
+    for k in range(1, ny + 1):
+        for j in range(1, nx + 1):
+            for i in range(1, nz + 1):
+                idx = L.fteik_model_grid2indexF(i, j, k, nz, nzx)
+                assert idx == (k - 1) * nzx + (j - 1) * nz + i
+                assert L.fteik_model_grid2index(i - 1, j - 1, k - 1, nz,
+                                                nzx) == idx - 1
+                L.fteik_model_index2gridF(idx, byref(i1), byref(j1),
+                                          byref(k1), byref(ierr))
+                assert ierr.value == 0
+                assert (i1.value, j1.value, k1.value) == (i, j, k)
+    assert L.fteik_model_velGrid2indexF(3, 4, 5, nz - 1, (nz - 1) * (nx - 1)) \
+        == 4 * (nz - 1) * (nx - 1) + 3 * (nz - 1) + 3
+    # past the last node: k out of range
+    L.fteik_model_index2gridF(nzx * ny + 1, byref(i1), byref(j1), byref(k1),
+                              byref(ierr))
+    assert ierr.value == 1
+    # the 2D solver's geometry: ny = 1
+    L.fteik_solver2d_initialize(12, 9, 0.0, 0.0, 5.0, 5.0, 1, 3.0, 0.0, 0,
+                                byref(ierr))
+    assert ierr.value == 0
+    L.fteik_model_index2gridF(12 * 9, byref(i1), byref(j1), byref(k1),
+                              byref(ierr))
+    assert ierr.value == 0 and (i1.value, j1.value, k1.value) == (12, 9, 1)
+    L.fteik_model_index2gridF(12 * 9 + 1, byref(i1), byref(j1), byref(k1),
+                              byref(ierr))
+    assert ierr.value == 1
+    L.fteik_solver2d_free()
+    L.fteik_model_index2gridF(5, byref(i1), byref(j1), byref(k1),
+                              byref(ierr))
+    assert ierr.value == 3
+    del g
+
+
 def test_multiple_sources_and_origin():
     """Several sources, solved one after another on the same state, with a
     non-zero grid origin (source.F90:136-190)."""
@@ -364,3 +434,31 @@ def test_sweep_overlap_matches_oracle(case, precision):
         _announce("chain tile", tile, "precision", precision, "case", case)
         assert g.solve(1) == 0
         np.testing.assert_array_equal(g.field(), plain)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_keeps_a_shape_that_matches_oracle(precision):
+    """fteik_solver3d_autotune times every usable tile shape on the grid,
+    keeps the fastest and leaves the solved field: it equals the oracle's
+    (FP64), and a solve with the chosen shape gives the same field."""
+    nz, nx, ny, h = 45, 52, 38, 15.0
+    vel = hetero(nz, nx, ny, h, 18)
+    zs, xs, ys = 30.2 * h, 8.4 * h, 20.7 * h
+    ref, _, _ = oracle3d(vel, nz, nx, ny, h, zs, xs, ys, 3, 5.0)
+    g = GPU3D(nz, nx, ny, h, nsweep=3, precision=precision)
+    g.set_cells(vel)
+    g.set_sources(zs, xs, ys)
+    L = lib()
+    t, t32, nt, ierr = c_int(-1), c_int(-1), c_int(), c_int(1)
+    L.fteik_solver3d_autotune(1, byref(t), byref(t32), byref(ierr))
+    assert ierr.value == 0
+    _announce("autotuned tiles", t.value, t32.value, "precision", precision)
+    cur, cur32 = c_int(), c_int()
+    L.fteik_solver3d_getTile(byref(cur), byref(cur32), byref(nt), byref(ierr))
+    assert (cur.value, cur32.value) == (t.value, t32.value)
+    assert 0 <= t.value < nt.value
+    tuned = g.field()
+    if precision == 0:
+        np.testing.assert_array_equal(tuned, ref)
+    assert g.solve(1) == 0
+    np.testing.assert_array_equal(g.field(), tuned)

@@ -179,6 +179,9 @@ def test_solve2d_matches_reference(fixtures, case):
                             slow=slow, is3d=False)
     tt = orc.solve2d(slow, nz, nx, dz, dx, src, nsweep, eps)
     np.testing.assert_array_equal(tt, fixtures["t2d_%s_tt" % case])
+    # the reference's FSM driver (solver2D.F90:1042-1280) reaches the same
+    # field: it updates the same masked nodes in a lexicographic order
+    np.testing.assert_array_equal(tt, fixtures["t2d_%s_fsm_tt" % case])
     zr, xr = fixtures["t2d_%s_rec" % case]
     ttr = orc.receivers(tt, (nz, nx, 1, dz, dx, 1.0, 0.0, 0.0, 0.0), zr, xr,
                         is3d=False)
