@@ -125,6 +125,7 @@ def load_library(path=None):
         "locate_setObservation64f": [i, i, c_bool, d, _ip, _dp, _dp, _ip],
         "locate_locateEvent": [i, _ip, _dp, _dp],
         "locate_locateEvents": [i, _ip, _ip, _dp, _dp, _ip],
+        "locate_locateEventsNoAlloc": [i, _ip, _ip, _dp, _dp, _ip],
         "locate_setReferenceCompatibility": [i, _ip],
         "fteik_analytic_initialize64f": [i, i, i, d, d, d, d, d, d, i, _ip],
         "fteik_analytic_setSources64f": [i, _dp, _dp, _dp, _ip],

@@ -19,7 +19,7 @@ OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libfteik_shared.so")
 SOURCES = ["capi_common.cpp", "capi3d.cpp", "capi2d.cpp", "capi_locate.cpp",
            "capi_model.cpp", "capi_batch.cpp", "capi_surrogate.cpp",
-           "capi_rays.cpp",
+           "capi_rays.cpp", "locate_noalloc.hip",
            "sweep3d.hip", "sweep2d.hip", "util.hip", "locate.hip",
            "stencil_eval.hip", "batch3d.hip", "surrogate.hip", "rays.hip"]
 

@@ -265,6 +265,16 @@ void locate_setReferenceCompatibility(int mode, int *ierr);
  * tables, each with a fresh origin-time estimate. */
 void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
                          double *t0Opt, double *objOpt, int *ierr);
+/* EXTENSION (no counterpart in the reference): locate_locateEvents without
+ * any device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Per-call
+ * device buffers come from one static device array of about 3.4 MB per GPU,
+ * held until the process ends: at most 4096 events per call and at most 64
+ * padded observations per event (ntf <= 64), else ierr = 1.  Results equal
+ * locate_locateEvents'.  Calls that overlap on one GPU share the array and
+ * race. */
+void locate_locateEventsNoAlloc(int n, const int *evnmbr, int *optIndx,
+                                double *t0Opt, double *objOpt, int *ierr);
 /* ========================= Batched 3D solver ========================= */
 /* EXTENSION (no counterpart in the reference, which solves one source at a
  * time, solver3D.F90:1156): a batch of ngrids independent single-source
