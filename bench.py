@@ -87,7 +87,7 @@ def batch_models(ngrids=64, h=50.0):
     return out
 
 
-def run_batch(L, models, h, nsweep, tile):
+def batch_setup(L, models, h, precision):
     ierr = c_int(1)
     dp = ctypes.POINTER(c_double)
     L.fteik_batch3d_initialize(len(models), byref(ierr))
@@ -97,41 +97,109 @@ def run_batch(L, models, h, nsweep, tile):
                                 vel.ctypes.data_as(dp), zs, xs, ys,
                                 byref(ierr))
         assert ierr.value == 0, "batch grid %d" % k
-    L.fteik_batch3d_setTile(tile, byref(ierr))
-    t0 = time.perf_counter()
-    L.fteik_batch3d_solve(nsweep, 5.0, byref(ierr))
-    wall = time.perf_counter() - t0
-    assert ierr.value == 0, "batch solve"
-    ms, upd, corr = c_double(), c_double(), c_int()
+    L.fteik_batch3d_setPrecision(precision, byref(ierr))
+    L.fteik_batch3d_setMixedParameters(1.0 / 64.0, 64, byref(ierr))
+    assert ierr.value == 0
+
+
+def batch_info(L):
+    ms, upd, corr, ierr = c_double(), c_double(), c_int(), c_int()
     L.fteik_batch3d_getLastSolveInfo(byref(ms), byref(corr), byref(upd),
                                      byref(ierr))
-    return ms.value, upd.value, wall
+    return ms.value, upd.value, corr.value
 
 
-def batch_section(L, ntiles):
-    """Batched solve: 64 independent grids of uneven size swept together; the
-    tile shape is picked by timing every shape on the batch (autotuner)."""
+def batch_fields(L, models):
+    ierr = c_int(1)
+    out = []
+    for k, ((nz, nx, ny), _, _) in enumerate(models):
+        tt = np.zeros(nz * nx * ny)
+        L.fteik_batch3d_getTravelTimeField64f(
+            k + 1, tt.size, 0, tt.ctypes.data_as(ctypes.POINTER(c_double)),
+            byref(ierr))
+        assert ierr.value == 0
+        out.append(tt)
+    return out
+
+
+def batch_section(L, args):
+    """Batched solve of 64 independent grids of uneven size in FP32-mixed
+    precision (FP32 iterations, then FP64 corrections to convergence), with
+    the tile shape chosen by the library's autotuner on the batch.  The
+    FP64 batch is solved to convergence too (the iteration count after
+    which one more changes no node): it is the parity reference of the
+    mixed fields (max relative error per grid), and a seeded sample of its
+    grids at nsweep = 2 is checked bitwise against the oracle."""
     h = 50.0
     models = batch_models()
-    times = {}
     ierr = c_int(1)
-    L.fteik_batch3d_initialize(1, byref(ierr))
-    for tile in range(ntiles):
-        L.fteik_batch3d_setTile(tile, byref(ierr))
-        if ierr.value != 0:          # shapes the batched solver does not use
-            continue
-        ms, upd, _ = run_batch(L, models, h, 2, tile)
-        times[tile] = ms
-    best = min(times, key=times.get)
-    ms, upd, wall = run_batch(L, models, h, 2, best)
+    # FP64 iterations to convergence (default tile)
+    prev, nconv = None, None
+    for ns in range(2, 33, 2):
+        batch_setup(L, models, h, 0)
+        L.fteik_batch3d_solve(ns, 5.0, byref(ierr))
+        assert ierr.value == 0
+        cur = batch_fields(L, models)
+        if prev is not None and all(np.array_equal(a, b)
+                                    for a, b in zip(cur, prev)):
+            nconv = ns - 2
+            break
+        prev = cur
+        L.fteik_batch3d_free()
+    if nconv is None:
+        nconv = 32
+    conv = prev
+    # FP64 at nconv, autotuned, timed
+    batch_setup(L, models, h, 0)
+    t64 = c_int(-1)
+    L.fteik_batch3d_autotune(nconv, 5.0, byref(t64), byref(ierr))
+    assert ierr.value == 0, "batch autotune (FP64)"
+    ms64, upd64, _ = batch_info(L)
     L.fteik_batch3d_free()
-    return {"workload": "64 grids, each axis U{64..192}, h=50 m, "
-                        "v=2000+U(0.3,0.8)z, +-3% smoothed, one source each, "
-                        "FP64, nsweep=2",
-            "grids": len(models), "updates": upd, "ms": ms,
-            "value": upd / (ms * 1e-3) / 1e9, "unit": "Gupdates/s",
-            "autotuned_tile": best,
-            "ms_by_tile": {str(k): v for k, v in times.items()}}
+    # FP32-mixed, autotuned, timed
+    batch_setup(L, models, h, 1)
+    tmx = c_int(-1)
+    L.fteik_batch3d_autotune(nconv, 5.0, byref(tmx), byref(ierr))
+    assert ierr.value == 0, "batch autotune (mixed)"
+    L.fteik_batch3d_solve(nconv, 5.0, byref(ierr))
+    assert ierr.value == 0, "batch mixed solve"
+    msmx, updmx, corr = batch_info(L)
+    mixed = batch_fields(L, models)
+    rel = [float(np.max(np.abs(m - c) / np.maximum(c, 1e-12)))
+           for m, c in zip(mixed, conv)]
+    L.fteik_batch3d_free()
+    out = {"workload": "64 grids, each axis U{64..192}, h=50 m, "
+                       "v=2000+U(0.3,0.8)z, +-3% smoothed, one source each, "
+                       "FP32-mixed, autotuned tile",
+           "grids": len(models), "fp64_iterations_to_converge": nconv,
+           "value": upd64 / (msmx * 1e-3) / 1e9, "unit": "Gupdates/s",
+           "ms": msmx, "autotuned_tile": tmx.value,
+           "fp64_correction_iterations": corr,
+           "updates_performed": updmx,
+           "mixed_max_rel_err_vs_fp64_per_grid": max(rel),
+           "grids_within_1e-6": int(sum(r <= 1e-6 for r in rel)),
+           "fp64": {"value": upd64 / (ms64 * 1e-3) / 1e9, "ms": ms64,
+                    "autotuned_tile": t64.value,
+                    "updates": upd64},
+           "speedup_mixed_vs_fp64": ms64 / msmx}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        batch_setup(L, models, h, 0)
+        L.fteik_batch3d_solve(2, 5.0, byref(ierr))
+        f2 = batch_fields(L, models)
+        L.fteik_batch3d_free()
+        same = True
+        for k in np.random.default_rng(55).choice(len(models), 2,
+                                                  replace=False):
+            (nz, nx, ny), vel, (zs, xs, ys) = models[k]
+            slow = orc.cell_slowness(vel, nz, nx, ny)
+            src = orc.source_locate((nz, nx, ny, h, h, h, 0.0, 0.0, 0.0), zs,
+                                    xs, ys, slow)
+            ref = orc.solve3d(slow, nz, nx, ny, h, h, h, src, 2, 5.0,
+                              schedule=1)
+            same = same and bool(np.array_equal(ref, f2[k]))
+        out["fp64_sample_equals_oracle"] = same
+    return out
 
 
 def locator_workload():
@@ -210,9 +278,17 @@ def solver2d_section(L, nsrc=256, cpu_sources=4):
         same = same and bool(np.array_equal(
             ttr[k], orc.receivers(ref, grid, zr, xr, is3d=False)))
     tcpu = time.perf_counter() - t0
-    out["cpu_baseline"] = {"value": cpu_sources / tcpu, "unit": "sources/s",
-                           "cores": 1, "kind": "port",
-                           "sample": "%d sources" % cpu_sources}
+    port = cpu_sources / tcpu
+    refr = ref_solver2d_rate(nz, nx, h, vel, zs[:cpu_sources],
+                             xs[:cpu_sources], 2, 3.0)
+    out["cpu_baseline"] = {"value": refr if refr else port,
+                           "unit": "sources/s", "cores": 1,
+                           "kind": "reference" if refr else "port",
+                           "cpu_model": cpu_model(),
+                           "sample": "%d sources" % cpu_sources,
+                           "port_value": port,
+                           "reference_over_port": (refr / port
+                                                   if refr else None)}
     out["gpu_equals_oracle_on_sample"] = same
     return out
 
@@ -222,25 +298,37 @@ def locator_section(L, args, nev=10000, cpu_events=16):
     solver and handed to the locator on the device."""
     (nz, nx, ny, h), vel, stations = locator_workload()
     ngrd = nz * nx * ny
-    models = [((nz, nx, ny), vel, s) for s in stations]
-    ms_tab, _, _ = run_batch(L, models, h, 2, 6)
     ierr = c_int(1)
     dp = ctypes.POINTER(c_double)
     ip = ctypes.POINTER(c_int)
     nsta = len(stations)
+    # station tables: the reciprocal station-as-source driver, batched on
+    # the 3D solver's grid and model, handed to the locator on the device
+    L.fteik_solver3d_initialize(nz, nx, ny, 0.0, 0.0, 0.0, h, h, h, 2, 5.0,
+                                0.0, 0, byref(ierr))
+    assert ierr.value == 0
+    L.fteik_solver3d_setVelocityModel64f(vel.size, vel.ctypes.data_as(dp),
+                                         byref(ierr))
+    assert ierr.value == 0
     L.locate_initialize(nev, nsta, ngrd, byref(ierr))
     assert ierr.value == 0
     L.locate_setReferenceCompatibility(0, byref(ierr))
+    sz, sx, sy = (np.ascontiguousarray([s[i] for s in stations])
+                  for i in range(3))
+    L.locate_setTravelTimeFieldsFromStations3d(
+        nsta, sz.ctypes.data_as(dp), sx.ctypes.data_as(dp),
+        sy.ctypes.data_as(dp), 2, 5.0, 0, 1, byref(ierr))
+    assert ierr.value == 0, "station tables"
+    ms_tab, _, _ = batch_info(L)
     tables = np.zeros((nsta, ngrd))
     for k in range(nsta):
-        L.locate_setTravelTimeFieldFromBatch3d(k + 1, k + 1, byref(ierr))
-        assert ierr.value == 0
         tt = np.zeros(ngrd)
         L.fteik_batch3d_getTravelTimeField64f(k + 1, ngrd, 0,
                                               tt.ctypes.data_as(dp),
                                               byref(ierr))
         tables[k] = tt
     L.fteik_batch3d_free()
+    L.fteik_solver3d_free()
     hypo = np.random.default_rng(10).integers(0, ngrd, nev)
     t0 = np.random.default_rng(11).uniform(0.0, 100.0, nev)
     noise = np.random.default_rng(12).normal(0.0, 0.05, (nev, nsta))
@@ -276,8 +364,7 @@ def locator_section(L, args, nev=10000, cpu_events=16):
                np.abs(opt - 1 - hypo) == 0))}
     # CPU baseline and parity on a sample of events (oracle, fresh state)
     from oracle import oracle as orc
-    cores = min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(cores)
+    cores = cpu_cores()
     loc = orc.Locator(nev, nsta, ngrd)
     for k in range(nsta):
         loc.set_table(k + 1, tables[k])
@@ -289,9 +376,16 @@ def locator_section(L, args, nev=10000, cpu_events=16):
         r = loc.locate(e + 1, accumulate_t0=False, pad_argmin=False)
         tcpu += time.perf_counter() - tstart
         same += r == (int(opt[e]), float(t0o[e]), float(obj[e]))
-    out["cpu_baseline"] = {"value": cpu_events / tcpu, "unit": "events/s",
-                           "cores": cores, "kind": "port",
-                           "sample": "%d events" % cpu_events}
+    port = cpu_events / tcpu
+    refr = ref_locator_rate(tables, picks, cpu_events, cores)
+    out["cpu_baseline"] = {"value": refr if refr else port,
+                           "unit": "events/s", "cores": cores,
+                           "kind": "reference" if refr else "port",
+                           "cpu_model": cpu_model(),
+                           "sample": "%d events" % cpu_events,
+                           "port_value": port,
+                           "reference_over_port": (refr / port
+                                                   if refr else None)}
     out["gpu_equals_oracle_on_sample"] = same == cpu_events
     out["surrogate"] = surrogate_section(L, stations, (nz, nx, ny, h), nev,
                                          args, opt, tgpu)
@@ -353,6 +447,106 @@ def surrogate_section(L, stations, grid, nev, args, full_opt, full_s):
             "fallback_events": nfb.value}
 
 
+def ref_locator_rate(tables, picks, nsample, cores):
+    """events/s of the reference's own OpenMP grid search
+    (locate.F90:112-126, :620-744) built from its sources (oracle/_ref),
+    on the first nsample events; None when the build is absent."""
+    from oracle import oracle as orc
+    if not orc.have_ref():
+        return None
+    from ctypes import c_bool
+    R = orc.ref()
+    dp = ctypes.POINTER(c_double)
+    ip = ctypes.POINTER(c_int)
+    nsta, ngrd = tables.shape
+    ierr = c_int(1)
+    R.locate_initialize(c_int(nsample), c_int(nsta), c_int(ngrd), byref(ierr))
+    if ierr.value != 0:
+        return None
+    for k in range(nsta):
+        t = np.ascontiguousarray(tables[k])
+        R.locate_setTravelTimeField64f(c_int(ngrd), c_int(k + 1),
+                                       t.ctypes.data_as(dp), byref(ierr))
+    o2t = np.arange(1, nsta + 1, dtype=np.int32)
+    w = np.ones(nsta)
+    for e in range(nsample):
+        p = np.ascontiguousarray(picks[e])
+        R.locate_setObservation64f(c_int(e + 1), c_int(nsta), c_bool(False),
+                                   c_double(0.0), o2t.ctypes.data_as(ip),
+                                   p.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                   byref(ierr))
+    opt, t0, ob = c_int(), c_double(), c_double()
+    tstart = time.perf_counter()
+    for e in range(nsample):
+        R.locate_locateEvent(c_int(e + 1), byref(opt), byref(t0), byref(ob))
+    sec = time.perf_counter() - tstart
+    R.locate_finalize()
+    return nsample / sec
+
+
+def ref_solver2d_rate(nz, nx, h, vel, zs, xs, nsweep, eps):
+    """sources/s of the reference's own 2D level-set solver (single-
+    threaded, solver2D.F90:782-978) built from its sources (oracle/_ref);
+    None when the build is absent."""
+    from oracle import oracle as orc
+    if not orc.have_ref():
+        return None
+    R = orc.ref()
+    dp = ctypes.POINTER(c_double)
+    ierr = c_int(1)
+    R.fteik_solver2d_initialize(c_int(nz), c_int(nx), c_double(0.0),
+                                c_double(0.0), c_double(h), c_double(h),
+                                c_int(nsweep), c_double(eps), c_double(0.0),
+                                c_int(0), byref(ierr))
+    if ierr.value != 0:
+        return None
+    v = np.ascontiguousarray(vel)
+    R.fteik_solver2d_setCellVelocityModel64f(c_int(v.size), c_int(0),
+                                             v.ctypes.data_as(dp), byref(ierr))
+    z, x = np.ascontiguousarray(zs), np.ascontiguousarray(xs)
+    R.fteik_solver2d_setSources64f(c_int(z.size), z.ctypes.data_as(dp),
+                                   x.ctypes.data_as(dp), byref(ierr))
+    tstart = time.perf_counter()
+    for k in range(z.size):
+        R.fteik_solver2d_solveSourceLSM(c_int(k + 1), byref(ierr))
+    sec = time.perf_counter() - tstart
+    R.fteik_solver2d_free()
+    return z.size / sec if ierr.value == 0 else None
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu), for the baseline records."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+_CORES = None
+
+
+def cpu_cores():
+    """Host threads for the CPU baselines: the cores this process may run
+    on at start-up, at most 16 (the host share of one GPU on the benchmark
+    machines).  Taken once, before any OpenMP runtime starts: a runtime
+    that binds its threads narrows the main thread's affinity, which a
+    later query would report."""
+    global _CORES
+    if _CORES is None:
+        try:
+            n = len(os.sched_getaffinity(0))
+        except AttributeError:
+            n = os.cpu_count() or 1
+        _CORES = max(1, min(16, n))
+    return _CORES
+
+
 def hip_sync():
     hip = ctypes.CDLL("libamdhip64.so.7")
     return hip.hipDeviceSynchronize()
@@ -373,6 +567,11 @@ def main():
                     help="run only the locator and surrogate sections "
                          "(for profiling)")
     args = ap.parse_args()
+    # one thread count for every OpenMP runtime the baselines load (the
+    # oracle's and the reference build's), fixed before either starts; no
+    # thread binding, so the reference's runtime, which starts later, still
+    # sees every core
+    os.environ["OMP_NUM_THREADS"] = str(cpu_cores())
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
               "supported (the solver has no multi-GPU path)" % args.gpus,
@@ -493,6 +692,14 @@ def main():
           flush=True)
     ms_launch32 = sweep_ms(1)
 
+    # latency floor of a sweep: one level after another, each at least one
+    # dependent iteration stencil (one wave per SIMD)
+    us_chain = c_double()
+    L.fteik_amd_timeStencilChain3d(0, 4000, byref(us_chain), byref(ierr))
+    assert ierr.value == 0, "stencil chain"
+    levels = 3 * (N - 1) - 2             # flipped i+j+k over the sweep box
+    floor_ms = levels * us_chain.value * 1e-3
+
     units = float((N - 1) ** 3)          # updates per Gauss-Seidel launch
     t1, t2, nt1 = c_int(), c_int(), c_int()
     L.fteik_solver3d_getTile(byref(t1), byref(t2), byref(nt1), byref(ierr))
@@ -506,9 +713,7 @@ def main():
     # ------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        os.environ["OMP_NUM_THREADS"] = str(cores)
-        os.environ.setdefault("OMP_PROC_BIND", "close")
+        cores = cpu_cores()
         from oracle import oracle as orc
         slow = orc.cell_slowness(vel, N, N, N)
         src = orc.source_locate((N, N, N, H, H, H, 0.0, 0.0, 0.0), zs, xs, ys,
@@ -518,7 +723,7 @@ def main():
                           schedule=1)
         tcpu = time.perf_counter() - t0
         cpu = {"value": upd64 / tcpu / 1e9, "unit": "Gupdates/s",
-               "cores": cores, "kind": "port",
+               "cores": cores, "kind": "port", "cpu_model": cpu_model(),
                "sample": "one complete FP64 solve of the same 257^3 "
                          "workload (oracle, OpenMP level-synchronous)",
                "seconds": tcpu,
@@ -564,14 +769,20 @@ def main():
                      "traffic": traffic,
                      "ms_per_launch": ms_launch64,
                      "bytes_per_update": BYTES_PER_UPDATE["f64"],
-                     "updates_per_launch": units},
+                     "updates_per_launch": units,
+                     "critical_path": {
+                         "levels_per_launch": levels,
+                         "stencil_chain_us_per_level": us_chain.value,
+                         "floor_ms_per_launch": floor_ms,
+                         "frac": floor_ms / ms_launch64,
+                         "note": "a sweep's levels are sequential; each "
+                                 "costs at least one dependent FP64 "
+                                 "iteration stencil (measured live, one "
+                                 "wave per SIMD)"}},
         "cpu_baseline": cpu,
     }
     if not args.no_extra:
-        nt = c_int()
-        t1, t2 = c_int(), c_int()
-        L.fteik_solver3d_getTile(byref(t1), byref(t2), byref(nt), byref(ierr))
-        out["batch"] = batch_section(L, nt.value)
+        out["batch"] = batch_section(L, args)
         print("bench: batch %.1f Gupd/s" % out["batch"]["value"],
               file=sys.stderr, flush=True)
         out["locator"] = locator_section(L, args)

@@ -147,6 +147,7 @@ def load_library(path=None):
         "fteik_amd_evalLocalSolver3d": [i, _dp, _ip, _dp, _dp, _ip],
         "fteik_amd_prepareSource3d": [i, _ip],
         "fteik_amd_timeSweeps3d": [i, i, _dp, _ip],
+        "fteik_amd_timeStencilChain3d": [i, i, _dp, _ip],
         "fteik_batch3d_initialize": [i, _ip],
         "fteik_batch3d_free": [],
         "fteik_batch3d_setGrid": [i, i, i, i, d, d, d, d, d, d, _dp, d, d, d,
@@ -159,6 +160,8 @@ def load_library(path=None):
         "fteik_batch3d_getTravelTimeField64f": [i, i, i, _dp, _ip],
         "fteik_batch3d_getNumberOfGrids": [_ip, _ip],
         "locate_setTravelTimeFieldFromBatch3d": [i, i, _ip],
+        "locate_setTravelTimeFieldsFromStations3d": [i, _dp, _dp, _dp, i, d,
+                                                     i, i, _ip],
         "fteik_surrogate_initialize": [i, i, i, i, d, d, d, _dp, _dp, _dp, d,
                                        d, i, _ip],
         "fteik_surrogate_free": [],

@@ -66,7 +66,7 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
 // lower-numbered ones and in-order dispatch keeps the lowest unfinished one
 // resident (the launch is larger than the GPU).  args[g]: grid g's sweep
 // arguments for this launch; each grid has its own mailbox.
-template <typename R, bool INIT, int BX, int BY, int HB, int MODE>
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE, int KB>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
 void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
@@ -81,8 +81,8 @@ void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
 #else   // the host pass only parses the kernel
     const SweepArgs<R> a = args[map[2*blockIdx.x]];
 #endif
-    pencil_mb_body<R, INIT, BX, BY, HB, MODE, false>(a, dyn_of(a),
-                                                     map[2*blockIdx.x + 1]);
+    pencil_mb_body<R, INIT, BX, BY, HB, MODE, false, false, KB>(
+        a, dyn_of(a), map[2*blockIdx.x + 1]);
 }
 
 // Sweep arguments of every grid for one mailbox sweep (linit: an
@@ -126,14 +126,16 @@ template <typename R, bool INIT>
 static const void* batch_mb_fn(int tile)
 {
 #define BM(n) case n: return (const void*) batch_mb_kernel<R, INIT, \
-        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode>;
+        kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode, \
+        kTiles[n].kb>;
     switch (tile) {
         BM(7) BM(8) BM(9) BM(10) BM(11) BM(12) BM(13)
+        BM(14) BM(15) BM(16) BM(17)
         default: return nullptr;
     }
 #undef BM
 }
-static_assert(kTiles[7].mode == 1 && kTiles[13].mode == 2 && kNumTiles == 14,
+static_assert(kTiles[7].mode == 1 && kTiles[13].mode == 2 && kNumTiles == 18,
               "batch_mb_fn lists the mailbox tiles");
 
 template <typename R, bool INIT>

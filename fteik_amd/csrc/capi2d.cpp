@@ -486,8 +486,8 @@ void fteik_solver2d_traceRaysToPoints(int np, const double *xp,
     G.dz = g2.dz; G.dx = g2.dx; G.dy = 1.0;
     G.z0 = g2.z0; G.x0 = g2.x0; G.y0 = 0.0;
     const SrcInfo &s = g2.src[(size_t) (g2.srcNumber - 1)];
-    if (!trace_rays(G, g2.d_slow, nullptr, g2.d_gx, g2.d_gz, s.zsi, s.xsi, 1,
-                    np, xp, nullptr, zp, who))
+    if (!trace_rays(G, g2.d_slow, nullptr, g2.d_gx, g2.d_gz, s.ztrue,
+                    s.xtrue, 0.0, np, xp, nullptr, zp, who))
         *ierr = 1;
 }
 

@@ -195,6 +195,16 @@ bool prepare_source3d(int isrc, const char *who)
 
 } // namespace
 
+bool fteik_capi::solver3d_view(Solver3DView *v)
+{
+    if (!g3.init || !g3.haveModel) return false;
+    v->nz = g3.nz; v->nx = g3.nx; v->ny = g3.ny;
+    v->dz = g3.dz; v->dx = g3.dx; v->dy = g3.dy;
+    v->z0 = g3.z0; v->x0 = g3.x0; v->y0 = g3.y0;
+    v->d_slow = g3.d_slow;
+    return true;
+}
+
 extern "C" {
 
 void fteik_solver3d_free(void)
@@ -966,8 +976,8 @@ void fteik_solver3d_traceRaysToPoints(int np, const double *zp,
     G.dz = g3.dz; G.dx = g3.dx; G.dy = g3.dy;
     G.z0 = g3.z0; G.x0 = g3.x0; G.y0 = g3.y0;
     const SrcInfo &s = g3.src[(size_t) (g3.srcNumber - 1)];
-    if (!trace_rays(G, g3.d_slow, g3.d_grad, nullptr, nullptr, s.zsi, s.xsi,
-                    s.ysi, np, xp, yp, zp, who))
+    if (!trace_rays(G, g3.d_slow, g3.d_grad, nullptr, nullptr, s.ztrue,
+                    s.xtrue, s.ytrue, np, xp, yp, zp, who))
         *ierr = 1;
 }
 
@@ -1032,6 +1042,39 @@ void fteik_amd_evalLocalSolver3d(int n, const double *in, const int *node,
               d2h(outInit, d_o2, (size_t) n*sizeof(double));
     dfree(d_in); dfree(d_node); dfree(d_o1); dfree(d_o2);
     if (!ok) *ierr = 1;
+}
+
+void fteik_amd_timeStencilChain3d(int precision, int nsteps,
+                                  double *usPerStep, int *ierr)
+{
+    // Device time of one lane's dependent chain of nsteps iteration
+    // stencils, with one wave per SIMD (4 per CU) on the whole GPU: the
+    // latency floor of one sweep step.  Uses the mesh constants of the last
+    // prepared source.
+    static const char *who = "fteik_amd_timeStencilChain3d";
+    *ierr = 0;
+    *usPerStep = 0.0;
+    if (!g3.cstReady || nsteps < 1 || (precision != 0 && precision != 1)) {
+        msg("%s: needs a prepared source, nsteps >= 1, precision 0 or 1", who);
+        *ierr = 1;
+        return;
+    }
+    const int nwg = 4*std::max(1, g_dev.ncu);
+    double *d_out = nullptr;
+    bool ok = dalloc(&d_out, 64*(size_t) nwg);
+    // warm-up, then the timed launch
+    ok = ok && fteik_k_time_stencil_chain(g3.d_cst, 16, nwg, precision,
+                                          d_out, g_dev.stream) == 0;
+    hipEventRecord(g3.ev0, g_dev.stream);
+    ok = ok && fteik_k_time_stencil_chain(g3.d_cst, nsteps, nwg, precision,
+                                          d_out, g_dev.stream) == 0;
+    hipEventRecord(g3.ev1, g_dev.stream);
+    ok = ok && sync_ok();
+    float ms = 0.0f;
+    if (ok) hipEventElapsedTime(&ms, g3.ev0, g3.ev1);
+    dfree(d_out);
+    if (!ok) { msg("%s: GPU error", who); *ierr = 1; return; }
+    *usPerStep = 1000.0*(double) ms/(double) nsteps;
 }
 
 void fteik_amd_timeSweeps3d(int precision, int nrep, double *msPerLaunch,

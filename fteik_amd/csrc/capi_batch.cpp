@@ -281,23 +281,22 @@ void fteik_batch3d_initialize(int ngrids, int *ierr)
     gb.init = true;
 }
 
-void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
-                           double x0, double y0, double dz, double dx,
-                           double dy, const double *vel, double zs, double xs,
-                           double ys, int *ierr)
+// Grid igrid of the batch: geometry, source and the slowness, either from
+// host cell velocities (vel) or copied from a device slowness (d_slow).
+static bool batch_set_grid(const char *who, int igrid, int nz, int nx,
+                           int ny, double z0, double x0, double y0, double dz,
+                           double dx, double dy, const double *vel,
+                           const double *d_slow, double zs, double xs,
+                           double ys)
 {
-    static const char *who = "fteik_batch3d_setGrid";
-    *ierr = 0;
     gb.haveTimes = false;
     if (!gb.init || igrid < 1 || igrid > (int) gb.g.size()) {
         msg("%s: batch not initialized or grid %d out of range", who, igrid);
-        *ierr = 1;
-        return;
+        return false;
     }
     if (nz < 3 || nx < 3 || ny < 3 || dz <= 0.0 || dx <= 0.0 || dy <= 0.0) {
         msg("%s: grid %d: need n >= 3 and positive spacing", who, igrid);
-        *ierr = 1;
-        return;
+        return false;
     }
     BGrid &x = gb.g[(size_t) (igrid - 1)];
     dfree(x.d_slow); dfree(x.d_tt); dfree(x.d_slow32); dfree(x.d_tt32);
@@ -306,26 +305,42 @@ void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
     x.nz = nz; x.nx = nx; x.ny = ny;
     x.z0 = z0; x.x0 = x0; x.y0 = y0; x.dz = dz; x.dx = dx; x.dy = dy;
     if (locate_source(true, nz, nx, ny, dz, dx, dy, z0, x0, y0, zs, xs, ys,
-                      igrid, &x.src)) { *ierr = 1; return; }
+                      igrid, &x.src)) return false;
     if (x.src.zsi > nz - 2 || x.src.xsi > nx - 2 || x.src.ysi > ny - 2) {
         msg("%s: grid %d: source in the last cell along an axis", who, igrid);
-        *ierr = 1;
-        return;
+        return false;
     }
     const long ncell = (long) (nz - 1)*(nx - 1)*(ny - 1);
     const long ngrd = (long) nz*nx*ny;
     if (!dalloc(&x.d_slow, (size_t) ncell) || !dalloc(&x.d_tt, (size_t) ngrd) ||
         !dalloc((char **) &x.d_cst, 512) || !dalloc((char **) &x.d_cst32, 512)) {
         msg("%s: out of GPU memory", who);
-        *ierr = 1;
-        return;
+        return false;
     }
-    if (!set_model(who, true, nz, nx, ny, 0, (int) ncell, 0, vel, x.d_slow)) {
-        *ierr = 1;
-        return;
+    if (d_slow) {
+        if (hipMemcpyAsync(x.d_slow, d_slow, (size_t) ncell*sizeof(double),
+                           hipMemcpyDeviceToDevice, g_dev.stream)
+            != hipSuccess || !sync_ok()) {
+            msg("%s: GPU error", who);
+            return false;
+        }
+    } else if (!set_model(who, true, nz, nx, ny, 0, (int) ncell, 0, vel,
+                          x.d_slow)) {
+        return false;
     }
     x.set = true;
     gb.mapTile = -1;
+    return true;
+}
+
+void fteik_batch3d_setGrid(int igrid, int nz, int nx, int ny, double z0,
+                           double x0, double y0, double dz, double dx,
+                           double dy, const double *vel, double zs, double xs,
+                           double ys, int *ierr)
+{
+    *ierr = batch_set_grid("fteik_batch3d_setGrid", igrid, nz, nx, ny, z0,
+                           x0, y0, dz, dx, dy, vel, nullptr, zs, xs, ys)
+            ? 0 : 1;
 }
 
 void fteik_batch3d_setMixedParameters(double bias, int maxCorrections,
@@ -548,6 +563,58 @@ void fteik_batch3d_getNumberOfGrids(int *ngrids, int *ierr)
 {
     *ierr = gb.init ? 0 : 1;
     *ngrids = (int) gb.g.size();
+}
+
+// The reciprocal station-as-source driver (fteik.c:186-235, :264-326 with
+// lreciprocal: receivers become sources), batched and on the device: every
+// station of the 3D solver's grid and model is solved as a source in one
+// batch, and its field becomes locator table itf0 + s, rounded to single
+// precision as locate_setTravelTimeField64f rounds (locate.F90:336-339).
+// The batch stays solved, so the fields can still be read with
+// fteik_batch3d_getTravelTimeField64f.
+void locate_setTravelTimeFieldsFromStations3d(int nsta, const double *zsta,
+                                              const double *xsta,
+                                              const double *ysta, int nsweep,
+                                              double eps, int precision,
+                                              int itf0, int *ierr)
+{
+    static const char *who = "locate_setTravelTimeFieldsFromStations3d";
+    *ierr = 0;
+    Solver3DView v;
+    if (!solver3d_view(&v)) {
+        msg("%s: the 3D solver's grid and model are not set", who);
+        *ierr = 1;
+        return;
+    }
+    if (!gl.init || nsta < 1 || itf0 < 1 || itf0 + nsta - 1 > gl.ntf ||
+        (long) v.nz*v.nx*v.ny != gl.ngrd) {
+        msg("%s: locator not initialized for %d tables from %d on this grid",
+            who, nsta, itf0);
+        *ierr = 1;
+        return;
+    }
+    if (precision != 0 && precision != 1) {
+        msg("%s: precision must be 0 (FP64) or 1 (mixed)", who);
+        *ierr = 1;
+        return;
+    }
+    fteik_batch3d_initialize(nsta, ierr);
+    if (*ierr) return;
+    for (int s = 0; s < nsta; s++)
+        if (!batch_set_grid(who, s + 1, v.nz, v.nx, v.ny, v.z0, v.x0, v.y0,
+                            v.dz, v.dx, v.dy, nullptr, v.d_slow, zsta[s],
+                            xsta[s], ysta[s])) {
+            msg("%s: station %d", who, s + 1);
+            *ierr = 1;
+            return;
+        }
+    fteik_batch3d_setPrecision(precision, ierr);
+    fteik_batch3d_solve(nsweep, eps, ierr);
+    if (*ierr == 1) return;
+    const int solved = *ierr;          // 2: mixed corrections unconverged
+    for (int s = 0; s < nsta && *ierr != 1; s++)
+        locate_setTravelTimeFieldFromBatch3d(s + 1, itf0 + s, ierr);
+    if (*ierr == 0) *ierr = solved;
 }
 
 // Table itf of the locator from batch grid igrid, on the device: the solved

@@ -113,6 +113,15 @@ float weight_norm(const float *w, int lntf);
 void mask_skipped(float *o, float *w, float *w2, const unsigned char *sk,
                   size_t n);
 
+// The 3D solver's grid and device slowness (for drivers built on it).
+struct Solver3DView {
+    int nz = 0, nx = 0, ny = 0;
+    double dz = 0, dx = 0, dy = 0, z0 = 0, x0 = 0, y0 = 0;
+    const double *d_slow = nullptr;
+};
+// false when the 3D solver is not initialized or has no model
+bool solver3d_view(Solver3DView *v);
+
 // ------------------------------------------------------------------- rays
 // Geometry of a ray trace (2D: is3d false, ny = 1).
 struct RayGeom {
@@ -133,14 +142,15 @@ struct RayStore {
 };
 extern RayStore gr;
 
-// Traces rays from the np points (model coordinates) to the source node of
-// cell (zsi, xsi, ysi) through the device gradient (d_g3 in 3D, d_gx and
-// d_gz in 2D) and slowness; false (with messages naming `who`) if a
-// destination lies outside the model or a ray does not reach the source.
+// Traces rays from the np points (model coordinates) to the source at
+// (zsrc, xsrc, ysrc) (model coordinates) through the device gradient (d_g3
+// in 3D, d_gx and d_gz in 2D) and slowness; false (with messages naming
+// `who`) if a destination lies outside the model or a ray does not reach
+// the source.
 bool trace_rays(const RayGeom &G, const double *d_slow, const double *d_g3,
-                const double *d_gx, const double *d_gz, int zsi, int xsi,
-                int ysi, int np, const double *x, const double *y,
-                const double *z, const char *who);
+                const double *d_gx, const double *d_gz, double zsrc,
+                double xsrc, double ysrc, int np, const double *x,
+                const double *y, const double *z, const char *who);
 void rays_clear();
 
 } // namespace fteik_capi

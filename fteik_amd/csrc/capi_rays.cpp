@@ -37,9 +37,9 @@ static bool check_destinations(const RayGeom &G, int np, const double *x,
 }
 
 bool trace_rays(const RayGeom &G, const double *d_slow, const double *d_g3,
-                const double *d_gx, const double *d_gz, int zsi, int xsi,
-                int ysi, int np, const double *x, const double *y,
-                const double *z, const char *who)
+                const double *d_gx, const double *d_gz, double zsrc,
+                double xsrc, double ysrc, int np, const double *x,
+                const double *y, const double *z, const char *who)
 {
     gr.nrays = 0;
     gr.pts.clear(); gr.npts.clear(); gr.time.clear(); gr.status.clear();
@@ -64,10 +64,13 @@ bool trace_rays(const RayGeom &G, const double *d_slow, const double *d_g3,
     fteik::RayGrid rg;
     rg.is3d = G.is3d ? 1 : 0; rg.nz = G.nz; rg.nx = G.nx; rg.ny = G.ny;
     rg.dz = G.dz; rg.dx = G.dx; rg.dy = G.dy;
-    // the source node (source.F90:179-181: xsrc = (xsi - 1) dx + x0)
-    const double xs = (double) (xsi - 1)*G.dx;
-    const double ys = G.is3d ? (double) (ysi - 1)*G.dy : 0.0;
-    const double zs = (double) (zsi - 1)*G.dz;
+    // the true source position: the solved field's minimum, where the
+    // gradient vanishes (the reference's 2D tracer aims at the source
+    // cell's first node instead, rays.F90:588-594, which a ray whose source
+    // lies inside a cell never reaches)
+    const double xs = xsrc - G.x0;
+    const double ys = G.is3d ? ysrc - G.y0 : 0.0;
+    const double zs = zsrc - G.z0;
     double *d_start = nullptr, *d_pts = nullptr, *d_time = nullptr;
     int *d_n = nullptr, *d_st = nullptr;
     const size_t npt = (size_t) np*maxpts*3;

@@ -87,6 +87,7 @@ void free_surrogate()
     dfree(gs.d_zero);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     free_inv_work();
+    fteik_k_sur_release();
     gs = Surrogate();
 }
 

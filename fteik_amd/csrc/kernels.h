@@ -162,7 +162,10 @@ int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
                          const float* xnorm, int lntf, int nev,
                          unsigned long long* best, hipStream_t s);
 
-// stencil_eval.hip (diagnostics: device stencils on given node inputs)
+// stencil_eval.hip (diagnostics: device stencils on given node inputs,
+// and the latency of the stencil's dependency chain)
+int fteik_k_time_stencil_chain(const void* cst, int nsteps, int nwg,
+                               int fp32, void* out, hipStream_t stream);
 int fteik_k_eval3d(const double* in, const int* node, const void* cst,
                    int n, double* out_noinit, double* out_init,
                    hipStream_t stream);
@@ -192,6 +195,7 @@ int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
                       int K, const float* bias, int accumulate,
                       hipStream_t stream);
 int fteik_k_f2bf(const float* a, void* b, long n, hipStream_t stream);
+void fteik_k_sur_release(void);
 int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
                        float* z, void* h, long rows, int H, hipStream_t stream);
 int fteik_k_sur_silu(const float* z, void* h, long n, hipStream_t stream);

@@ -12,7 +12,7 @@
 // (four in 2D) nodes of the cell holding the point and u the cell's
 // slowness, in steps of ds = min(h)/2 by Euler, midpoint or classical RK4,
 // positions clamped into the grid, until a step lands within
-// sqrt(sum h^2 / 2) of the source node, where the ray is snapped.  The ray
+// sqrt(sum h^2 / 2) of the source, where the ray is snapped.  The ray
 // time accumulates |step| times the slowness of the cell each step starts
 // in.  Compiled without contraction, so a ray is bitwise the same as the
 // same arithmetic on the host.

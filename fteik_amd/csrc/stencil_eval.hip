@@ -30,7 +30,54 @@ __global__ void k_eval3d(const double* __restrict__ in,
                                 ty[sw]);
 }
 
+// Latency of the iteration stencil on its dependency chain: each lane runs
+// nsteps noinit3d evaluations, each one's neighbour times taken from the
+// previous result (a diagonal plane wave, so the 1D, 2D and 3D operators
+// all compete), as a column does from level to level in a sweep.  One
+// lane's chain is the lower bound of a sweep step.
+template <typename R>
+__global__ void k_stencil_chain(const fteik::Consts<double>* __restrict__ cd,
+                                int nsteps, R* __restrict__ out)
+{
+    const fteik::Consts<double>& a = *cd;
+    fteik::Consts<R> c;
+#define CV(f) c.f = (R) a.f
+    CV(dz); CV(dx); CV(dy); CV(dzi); CV(dxi); CV(dyi); CV(dz2i); CV(dx2i);
+    CV(dy2i); CV(dsum_nine); CV(dsumi); CV(epsSolver); CV(dz2dx2); CV(dz2dy2);
+    CV(dx2dy2); CV(dz2i_dx2i); CV(dz2i_dy2i); CV(dx2i_dy2i); CV(dz2i_p_dx2i);
+    CV(dz2i_p_dy2i); CV(dx2i_p_dy2i); CV(dz2i_p_dy2i_inv); CV(dz2i_p_dx2i_inv);
+    CV(dx2i_p_dy2i_inv); CV(szero); CV(szero2); CV(zsa); CV(xsa); CV(ysa);
+#undef CV
+    c.zsi = a.zsi; c.xsi = a.xsi; c.ysi = a.ysi; c.pad = 0;
+    const R h = c.dz, s0 = R(1)/R(3000);
+    fteik::Slw<R> s{s0, s0, s0, s0, s0, s0, s0};
+    R t = R(0.1) + R(1e-6)*(R) threadIdx.x;
+    const R d1 = h*s0*R(0.57735), d2 = h*s0*R(0.81650);
+    for (int k = 0; k < nsteps; k++) {
+        fteik::Nbr<R> n{t, t + R(1e-7), t + R(2e-7), t + d1, t + d1 + R(1e-7),
+                        t + d1 + R(2e-7), t + d2};
+        t = fteik::noinit3d(c, n, s, t + R(1));
+    }
+    out[blockIdx.x*blockDim.x + threadIdx.x] = t;
+}
+
 } // namespace
+
+// Times one dependent stencil chain per lane of nwg workgroups of one wave.
+extern "C" int fteik_k_time_stencil_chain(const void* cst, int nsteps,
+                                          int nwg, int fp32, void* out,
+                                          hipStream_t s)
+{
+    if (fp32)
+        hipLaunchKernelGGL(k_stencil_chain<float>, dim3(nwg), dim3(64), 0, s,
+                           (const fteik::Consts<double>*) cst, nsteps,
+                           (float*) out);
+    else
+        hipLaunchKernelGGL(k_stencil_chain<double>, dim3(nwg), dim3(64), 0, s,
+                           (const fteik::Consts<double>*) cst, nsteps,
+                           (double*) out);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 extern "C" int fteik_k_eval3d(const double* in, const int* node,
                               const void* cst, int n, double* out_noinit,

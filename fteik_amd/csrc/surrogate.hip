@@ -14,6 +14,7 @@
 // |x - station| / v0; the baseline takes most of the dynamic range.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include "kernels.h"
 
 namespace {
@@ -64,40 +65,80 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 //   C: lane l, register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 constexpr int GT = 64, GK = 32, GPAD = 8;
 
+// Staging of one operand tile [64 rows][32 k] into LDS as [row][k].  The
+// global reads follow the operand's layout: with k contiguous (sk == 1) a
+// thread reads 8 consecutive k of one row as one 16-byte load; with rows
+// contiguous (sr == 1, a transposed operand) it reads 8 consecutive rows
+// of one k, so a wave's reads are contiguous either way.
+__device__ __forceinline__ void stage_tile(const bf16_t* __restrict__ X,
+                                           long sr, long sk, long r0, int R,
+                                           int k0, int K,
+                                           bf16_t (*Xs)[GK + GPAD], int t)
+{
+    if (sk == 1) {
+        const int lr = t >> 2, lk = (t & 3)*8;
+        const long r = r0 + lr;
+        const int k = k0 + lk;
+        const bf16_t* p = X + r*sr + k;
+        if (r < R && k + 7 < K && (sr & 7) == 0 &&
+            (((uintptr_t) p) & 15) == 0) {
+            *(uint4*) &Xs[lr][lk] = *(const uint4*) p;
+        } else {
+            #pragma unroll
+            for (int q = 0; q < 8; q++)
+                Xs[lr][lk + q] = (r < R && k + q < K) ? p[q] : (bf16_t) 0;
+        }
+    } else if (sr == 1) {
+        const int lk = t & 31, lr = (t >> 5)*8;
+        const long r = r0 + lr;
+        const int k = k0 + lk;
+        const bf16_t* p = X + (long) k*sk + r;
+        bf16_t v[8];
+        if (k < K && r + 7 < R && (((uintptr_t) p) & 15) == 0) {
+            const uint4 w = *(const uint4*) p;
+            const bf16_t* wv = (const bf16_t*) &w;
+            #pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = wv[q];
+        } else {
+            #pragma unroll
+            for (int q = 0; q < 8; q++)
+                v[q] = (k < K && r + q < R) ? p[q] : (bf16_t) 0;
+        }
+        #pragma unroll
+        for (int q = 0; q < 8; q++) Xs[lr + q][lk] = v[q];
+    } else {
+        const int lr = t >> 2, lk = (t & 3)*8;
+        const long r = r0 + lr;
+        #pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int k = k0 + lk + q;
+            Xs[lr][lk + q] = (r < R && k < K) ? X[r*sr + (long) k*sk]
+                                              : (bf16_t) 0;
+        }
+    }
+}
+
+// One workgroup: the 64 x 64 tile (blockIdx.y, blockIdx.x) of C over the
+// k range of split blockIdx.z (kper values each).  nsplit == 1 writes C
+// (with bias, optionally accumulating); otherwise the partial sums go to
+// P[z][M][N] for k_gemm_reduce.
 __global__ __launch_bounds__(256)
 void k_gemm_bf16(const bf16_t* __restrict__ A, long sam, long sak,
                  const bf16_t* __restrict__ B, long sbk, long sbn,
                  float* __restrict__ C, long ldc, int M, int N, int K,
-                 const float* __restrict__ bias, int accumulate)
+                 const float* __restrict__ bias, int accumulate, int kper,
+                 float* __restrict__ P)
 {
     __shared__ __attribute__((aligned(16))) bf16_t As[GT][GK + GPAD];
     __shared__ __attribute__((aligned(16))) bf16_t Bs[GT][GK + GPAD];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
     const long m0 = (long) blockIdx.y*GT, n0 = (long) blockIdx.x*GT;
+    const int kb = blockIdx.z*kper, ke = min(K, kb + kper);
     f32x16 acc = {};
-    const int lr = t >> 2, lk = (t & 3)*8;       // loader: row, first k
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        // stage A rows m0..m0+63 and B columns n0..n0+63, k0..k0+31
-        {
-            const long m = m0 + lr;
-            bf16_t v[8];
-            #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int k = k0 + lk + q;
-                v[q] = (m < M && k < K) ? A[m*sam + (long) k*sak] : (bf16_t) 0;
-            }
-            #pragma unroll
-            for (int q = 0; q < 8; q++) As[lr][lk + q] = v[q];
-            const long n = n0 + lr;
-            #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int k = k0 + lk + q;
-                v[q] = (n < N && k < K) ? B[(long) k*sbk + n*sbn] : (bf16_t) 0;
-            }
-            #pragma unroll
-            for (int q = 0; q < 8; q++) Bs[lr][lk + q] = v[q];
-        }
+    for (int k0 = kb; k0 < ke; k0 += GK) {
+        stage_tile(A, sam, sak, m0, M, k0, ke, As, t);
+        stage_tile(B, sbn, sbk, n0, N, k0, ke, Bs, t);
         __syncthreads();
         #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
@@ -111,6 +152,15 @@ void k_gemm_bf16(const bf16_t* __restrict__ A, long sam, long sak,
     }
     const long n = n0 + wn*32 + (lane & 31);
     if (n >= N) return;
+    if (P) {
+        float* Pz = P + (long) blockIdx.z*M*N;
+        #pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const long m = m0 + wm*32 + 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+            if (m < M) Pz[m*N + n] = acc[r];
+        }
+        return;
+    }
     const float bn = bias ? bias[n] : 0.0f;
     #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -122,6 +172,25 @@ void k_gemm_bf16(const bf16_t* __restrict__ A, long sam, long sak,
         }
     }
 }
+
+// C = sum of the nsplit partial products (in split order) + bias.
+__global__ void k_gemm_reduce(const float* __restrict__ P, int nsplit, int M,
+                              int N, float* __restrict__ C, long ldc,
+                              const float* __restrict__ bias, int accumulate)
+{
+    GRID_STRIDE(i, (long) M*N) {
+        const long m = i / N, n = i - m*N;
+        float v = 0.0f;
+        for (int z = 0; z < nsplit; z++) v += P[(long) z*M*N + i];
+        v += bias ? bias[n] : 0.0f;
+        if (accumulate) v += C[m*ldc + n];
+        C[m*ldc + n] = v;
+    }
+}
+
+// workspace of the split-K partial products (grown on demand)
+float* g_split = nullptr;
+size_t g_split_cap = 0;
 
 // ------------------------------------------------------ elementwise parts
 // First layer on the VALU (K = 3): z = W1 x + b1, h = silu(z).
@@ -595,11 +664,44 @@ int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
                       hipStream_t s)
 {
     if (M < 1 || N < 1 || K < 1) return 0;
-    dim3 grid((N + GT - 1)/GT, (M + GT - 1)/GT);
+    const int tiles = ((N + GT - 1)/GT)*((M + GT - 1)/GT);
+    // few output tiles over a long k (the weight gradients: 256 x 256 over
+    // the batch rows): split k so the GPU's 256 CUs all work, then sum the
+    // partial products in a fixed order (the result does not depend on
+    // timing)
+    int nsplit = 1;
+    if (tiles < 256 && K >= 2048)
+        nsplit = std::min(std::max(512/tiles, 1), K/256);
+    const int kper = ((K + nsplit - 1)/nsplit + GK - 1)/GK*GK;
+    nsplit = (K + kper - 1)/kper;
+    float* P = nullptr;
+    if (nsplit > 1) {
+        const size_t need = (size_t) nsplit*M*N;
+        if (need > g_split_cap) {
+            if (g_split) hipFree(g_split);
+            g_split = nullptr;
+            g_split_cap = 0;
+            if (hipMalloc((void**) &g_split, need*sizeof(float)) != hipSuccess)
+                return 1;
+            g_split_cap = need;
+        }
+        P = g_split;
+    }
+    dim3 grid((N + GT - 1)/GT, (M + GT - 1)/GT, nsplit);
     hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s,
                        (const bf16_t*) A, sam, sak, (const bf16_t*) B, sbk,
-                       sbn, C, ldc, M, N, K, bias, accumulate);
+                       sbn, C, ldc, M, N, K, bias, accumulate, kper, P);
+    if (nsplit > 1)
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(nblocks((long) M*N)), dim3(TPB),
+                           0, s, P, nsplit, M, N, C, ldc, bias, accumulate);
     return ok();
+}
+
+void fteik_k_sur_release(void)
+{
+    if (g_split) hipFree(g_split);
+    g_split = nullptr;
+    g_split_cap = 0;
 }
 
 int fteik_k_f2bf(const float* a, void* b, long n, hipStream_t s)

@@ -37,7 +37,12 @@ namespace fteik {
 //         polls it ahead of the compute waves (pencil_mb_body);
 // mode 2: as mode 1, plus a store wave that takes the global stores off the
 //         compute waves.
-struct TileShape { int bx, by, hb, mode; };
+// kb: mailbox steps a halo lane fetches per round trip (edge-major
+//     mailbox, modes 1 and 2): it waits for the last step of the batch to
+//     be posted, then loads the batch's words together, so a tile trails
+//     its upwind neighbour by about kb steps but pays one mailbox round
+//     trip per kb steps instead of one per step.
+struct TileShape { int bx, by, hb, mode, kb = 1; };
 constexpr TileShape kTiles[] = {
     {16, 16, 8, 0}, {16, 16, 4, 0}, {16, 16, 2, 0}, {8, 8, 4, 0},
     {8, 8, 2, 0}, {32, 8, 4, 0}, {16, 8, 4, 0},
@@ -46,6 +51,8 @@ constexpr TileShape kTiles[] = {
     // 4 waves in at most 128 registers: four workgroups per CU, so two
     // sweeps' tiles of a 257^2 chain are resident together
     {16, 8, 2, 2},
+    // batched mailbox reads
+    {8, 8, 4, 2, 4}, {8, 8, 4, 2, 2}, {16, 8, 4, 2, 4}, {8, 8, 4, 2, 8},
 };
 // Waves per SIMD the kernels of a tile shape are compiled for.  The
 // hardware places a workgroup's waves on the SIMDs from the first one, so a
@@ -502,6 +509,76 @@ __device__ __forceinline__ bool mb_get(unsigned long long* mb, long g,
     return true;
 }
 
+// KB mailbox entries of this launch in one round trip: the loads are
+// issued together and waited for once (in one asm block, so no result
+// register is live while a load is in flight outside it).  ok[q] tells
+// whether entry q carried the tag.
+template <int KB>
+__device__ __forceinline__ void mb_get_batch(unsigned long long* mb,
+                                             const long (&g)[KB],
+                                             unsigned tag, double (&v)[KB],
+                                             bool (&ok)[KB])
+{
+    MbPair w[KB];
+    if constexpr (KB == 2) {
+        asm volatile("global_load_dwordx4 %0, %2, off sc1\n\t"
+                     "global_load_dwordx4 %1, %3, off sc1\n\t"
+                     "s_waitcnt vmcnt(0)"
+                     : "=&v"(w[0]), "=&v"(w[1])
+                     : "v"(mb + 2*g[0]), "v"(mb + 2*g[1]) : "memory");
+    } else if constexpr (KB == 4) {
+        asm volatile("global_load_dwordx4 %0, %4, off sc1\n\t"
+                     "global_load_dwordx4 %1, %5, off sc1\n\t"
+                     "global_load_dwordx4 %2, %6, off sc1\n\t"
+                     "global_load_dwordx4 %3, %7, off sc1\n\t"
+                     "s_waitcnt vmcnt(0)"
+                     : "=&v"(w[0]), "=&v"(w[1]), "=&v"(w[2]), "=&v"(w[3])
+                     : "v"(mb + 2*g[0]), "v"(mb + 2*g[1]),
+                       "v"(mb + 2*g[2]), "v"(mb + 2*g[3]) : "memory");
+    } else {
+        static_assert(KB == 8, "batches of 2, 4 or 8 entries");
+        asm volatile("global_load_dwordx4 %0, %8, off sc1\n\t"
+                     "global_load_dwordx4 %1, %9, off sc1\n\t"
+                     "global_load_dwordx4 %2, %10, off sc1\n\t"
+                     "global_load_dwordx4 %3, %11, off sc1\n\t"
+                     "global_load_dwordx4 %4, %12, off sc1\n\t"
+                     "global_load_dwordx4 %5, %13, off sc1\n\t"
+                     "global_load_dwordx4 %6, %14, off sc1\n\t"
+                     "global_load_dwordx4 %7, %15, off sc1\n\t"
+                     "s_waitcnt vmcnt(0)"
+                     : "=&v"(w[0]), "=&v"(w[1]), "=&v"(w[2]), "=&v"(w[3]),
+                       "=&v"(w[4]), "=&v"(w[5]), "=&v"(w[6]), "=&v"(w[7])
+                     : "v"(mb + 2*g[0]), "v"(mb + 2*g[1]),
+                       "v"(mb + 2*g[2]), "v"(mb + 2*g[3]),
+                       "v"(mb + 2*g[4]), "v"(mb + 2*g[5]),
+                       "v"(mb + 2*g[6]), "v"(mb + 2*g[7]) : "memory");
+    }
+    #pragma unroll
+    for (int q = 0; q < KB; q++) {
+        const unsigned long long lo = w[q].x, hi = w[q].y;
+        ok[q] = (unsigned) (lo >> 32) == tag && (unsigned) (hi >> 32) == tag;
+        v[q] = __longlong_as_double((long long) ((hi << 32) |
+                                                 (lo & 0xffffffffull)));
+    }
+}
+template <int KB>
+__device__ __forceinline__ void mb_get_batch(unsigned long long* mb,
+                                             const long (&g)[KB],
+                                             unsigned tag, float (&v)[KB],
+                                             bool (&ok)[KB])
+{
+    unsigned long long w[KB];
+    #pragma unroll
+    for (int q = 0; q < KB; q++)
+        w[q] = __hip_atomic_load(mb + g[q], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+    #pragma unroll
+    for (int q = 0; q < KB; q++) {
+        ok[q] = (unsigned) (w[q] >> 32) == tag;
+        v[q] = __uint_as_float((unsigned) (w[q] & 0xffffffffull));
+    }
+}
+
 __device__ __forceinline__ int lds_load(int* p)
 {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -540,11 +617,13 @@ __device__ __forceinline__ int lds_load(int* p)
 // steps it releases.  The store wave publishes the tile's own progress with
 // a release after its stores; field loads and stores are device-coherent.
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE,
-          bool SKIP, bool CHAIN = false>
+          bool SKIP, bool CHAIN = false, int KB = 1>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                                const SweepDyn& d,
                                                const int wg)
 {
+    static_assert(KB == 1 || (!CHAIN && 8 % KB == 0),
+                  "batched mailbox reads: edge-major mailbox, KB | 8");
     static_assert(!(INIT && SKIP), "tile skipping is for iteration sweeps");
     static_assert(!CHAIN || (!INIT && !SKIP && MODE == 2),
                   "sweep chains: iteration sweeps of mode-2 tiles");
@@ -786,10 +865,61 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             verify(s0 + 3*HD - 1);
             if (CHAIN && lds_load(&s_abort)) return;
             if (s0 + HD <= s_end) fetch(s0 + HD, nxt);
+            R mbv[KB > 1 ? KB : 1];
             #pragma unroll
             for (int q = 0; q < HD; q++) {
                 const int s = s0 + q;
                 if (s > s_end) break;                 // wave-uniform
+                if constexpr (KB > 1) {
+                    if (q % KB == 0 && !direct && hcol && a.mbx) {
+                        // this lane's steps of the batch s .. s + KB - 1
+                        const int lo = max(s, ez0 + hj + hk);
+                        const int hi = min(min(s + KB - 1, s_end),
+                                           a.fz1 + hj + hk);
+                        const bool xs = hl < BY || hl == BX + BY;
+                        auto addr = [&](int st) -> long {
+                            return xs ? ((long) (fbx - 1)*a.mbnt + st)*ny + hk
+                                      : ((long) (fby - 1)*a.mbnt + st)*nx + hj;
+                        };
+                        unsigned long long* mbp = xs ? a.mbx : a.mby;
+                        if (lo <= hi) {
+                            // the batch's last word first (the upwind tile
+                            // posts its steps in order), then all at once
+                            R vl;
+                            unsigned long long sp = 0;
+                            bool fail = false;
+                            while (!mb_get(mbp, addr(hi), tag, vl)) {
+                                __builtin_amdgcn_s_sleep(1);
+                                if (++sp > kSpinLimit) { fail = true; break; }
+                            }
+                            long gq[KB];
+                            bool okq[KB];
+                            #pragma unroll
+                            for (int qq = 0; qq < KB; qq++)
+                                gq[qq] = addr(min(max(s + qq, lo), hi));
+                            if (!fail) mb_get_batch<KB>(mbp, gq, tag, mbv, okq);
+                            #pragma unroll
+                            for (int qq = 0; qq < KB; qq++) {
+                                const int st = s + qq;
+                                if (fail || st < lo || st > hi || okq[qq])
+                                    continue;
+                                // not visible yet although a later step is
+                                sp = 0;
+                                while (!mb_get(mbp, gq[qq], tag, mbv[qq])) {
+                                    __builtin_amdgcn_s_sleep(1);
+                                    if (++sp > kSpinLimit) { fail = true;
+                                                             break; }
+                                }
+                            }
+                            if (fail) {
+                                atomicExch(a.err, 1);
+                                __hip_atomic_store(&s_abort, 1,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
+                        }
+                    }
+                }
                 // the ring slot of step s was last read at step s - RING + 3
                 const int need = s - RING + 3;
                 unsigned long long spins = 0;
@@ -800,7 +930,9 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 const int fi = s - hj - hk;
                 if (hcol && fi >= ez0 && fi <= a.fz1) {
                     R v = cur[q];
-                    if (!direct) {
+                    if (KB > 1 && !direct && a.mbx) {
+                        v = mbv[q % (KB > 1 ? KB : 1)];
+                    } else if (!direct) {
                         unsigned long long* mbp = a.mb;
                         long g = gcol + ((d.sz > 0) ? fi : nz - 1 - fi);
                         if (!CHAIN && a.mbx) {

@@ -275,6 +275,7 @@ void locate_locateEvents(int n, const int *evnmbr, int *optIndx,
  * race. */
 void locate_locateEventsNoAlloc(int n, const int *evnmbr, int *optIndx,
                                 double *t0Opt, double *objOpt, int *ierr);
+
 /* ========================= Batched 3D solver ========================= */
 /* EXTENSION (no counterpart in the reference, which solves one source at a
  * time, solver3D.F90:1156): a batch of ngrids independent single-source
@@ -310,6 +311,22 @@ void fteik_batch3d_getNumberOfGrids(int *ngrids, int *ierr);
 /* EXTENSION: locator table itf (1-based) from batch grid igrid without a
  * host round trip; same rounding as locate_setTravelTimeField64f. */
 void locate_setTravelTimeFieldFromBatch3d(int igrid, int itf, int *ierr);
+/* EXTENSION: the reciprocal station-as-source driver of fteik.c:186-235
+ * (with lreciprocal, receivers become sources), on the device: the nsta
+ * stations (model coordinates) are solved as sources on the 3D solver's
+ * grid and model in one batch (fteik_batch3d_*, precision 0 = FP64, 1 =
+ * mixed; nsweep, eps as for the solver) and station s's field becomes
+ * locator table itf0 + s (1-based), rounded as
+ * locate_setTravelTimeField64f rounds.  Needs fteik_solver3d_initialize +
+ * a velocity model, and locate_initialize with ngrd = nz*nx*ny and
+ * ntf >= itf0 + nsta - 1.  The batch is left solved (its fields can be
+ * read with fteik_batch3d_getTravelTimeField64f).  ierr = 2: the mixed
+ * mode's corrections did not converge (tables still set). */
+void locate_setTravelTimeFieldsFromStations3d(int nsta, const double *zsta,
+                                              const double *xsta,
+                                              const double *ysta, int nsweep,
+                                              double eps, int precision,
+                                              int itf0, int *ierr);
 
 /* ===================== Learned locator surrogate ===================== */
 /* EXTENSION (no counterpart in the reference): an MLP 3 -> 256 -> 256 ->
@@ -436,6 +453,13 @@ void fteik_amd_prepareSource3d(int isrc, int *ierr);
  * sweeps. */
 void fteik_amd_timeSweeps3d(int precision, int nrep, double *msPerLaunch,
                             int *ierr);
+/* EXTENSION: device time (microseconds per step) of one dependent chain of
+ * nsteps iteration stencils per lane, one wave per SIMD on the whole GPU,
+ * FP64 (precision 0) or FP32 (1): the latency floor of a sweep step,
+ * which bounds a sweep at (number of levels) x this.  Needs the mesh
+ * constants of a solved or prepared source. */
+void fteik_amd_timeStencilChain3d(int precision, int nsteps,
+                                  double *usPerStep, int *ierr);
 /* EXTENSION: library build string. */
 const char *fteik_amd_version(void);
 /* EXTENSION: releases every solver's device state, then the library's HIP

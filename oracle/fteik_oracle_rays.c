@@ -17,10 +17,11 @@
  * the point and u the cell's slowness; steps of ds = min(h)/2 by Euler,
  * midpoint or classical RK4 (the RK4 the comment at rays.F90:690-694
  * states); positions clamped into the grid (rays.F90:812-843); the ray ends
- * once a step lands within sqrt(sum h^2 / 2) of the source node (the
- * node of the source cell's first corner, source.F90:181), where it is
- * snapped; its time accumulates |step| * slowness of the cell the step
- * starts in (rays.F90:596-598).
+ * once a step lands within sqrt(sum h^2 / 2) of the source (the true
+ * position, where the field has its minimum: the reference's 2D tracer aims
+ * at the source cell's first node, rays.F90:588-594, which rays to a source
+ * inside a cell stall short of), where it is snapped; its time accumulates
+ * |step| * slowness of the cell the step starts in (rays.F90:596-598).
  */
 #include <math.h>
 #include "fteik_oracle.h"
@@ -216,7 +217,7 @@ static void ray_step(const ray_grid *G, int integrator, double ds,
     }
 }
 
-/* One ray from (xr, yr, zr) to the source node (xs, ys, zs), both
+/* One ray from (xr, yr, zr) to the source (xs, ys, zs), both
  * grid-relative metres.  pts receives up to maxpts points (x y z);
  * returns 0 when the ray reached the source, 1 otherwise. */
 static int trace_one(const ray_grid *G, int integrator, double xs, double ys,

@@ -263,6 +263,14 @@ def analytic_gradient(nz, nx, ny, dz, dx, dy, z0, x0, y0, zs, xs, ys, vtop,
     return out
 
 
+def analytic_gradient_point(nz, dz, z0, zs, xs, ys, vtop, vbot, z, x, y):
+    """analytic.F90:456-528 at one point (z, x, y)."""
+    f = lib().or_analytic_gradient_point
+    f.argtypes = [c_int] + [c_double] * 10
+    f.restype = c_double
+    return f(nz, dz, z0, zs, xs, ys, vtop, vbot, z, x, y)
+
+
 def xfast_to_natural(a, nz, nx, ny):
     """analytic.F90:722-731: [nz][ny][nx] x-fastest -> z-fastest."""
     return np.ascontiguousarray(

@@ -134,3 +134,25 @@ subroutine refh_orig_solver3d(vel, tt, nz, nx, ny, zs, xs, ys, dz, dx, dy, &
   end interface
   call FTeik(vel, tt, nz, nx, ny, zs, xs, ys, dz, dx, dy, nsweep, eps)
 end subroutine
+
+! solver2D.F90:984-1032 through the reference's own call path:
+! fteik_solver2d_traceRaysToPoints (solver2D.F90:310-355) differentiates the
+! solved field before it checks the ray destinations, so a call with one
+! destination outside the model computes tgradx/tgradz and returns with
+! ierr = 1 before any tracing.  The gradients are copied out unchanged.
+subroutine refh_solver2d_gradient(n, gx, gz, ierr) &
+           bind(c, name='refh_solver2d_gradient')
+  use iso_c_binding
+  use fteik2d_solver64f, only : fteik_solver2d_traceRaysToPoints, &
+                                tgradx, tgradz
+  implicit none
+  integer(c_int), value :: n
+  real(c_double), intent(out) :: gx(n), gz(n)
+  integer(c_int), intent(out) :: ierr
+  real(c_double) :: xp(1), zp(1)
+  xp(1) = -1.d30
+  zp(1) = -1.d30
+  call fteik_solver2d_traceRaysToPoints(1, xp, zp, ierr)
+  gx(1:n) = tgradx(1:n)
+  gz(1:n) = tgradz(1:n)
+end subroutine

@@ -269,6 +269,32 @@ def solve2d_fsm_ref(nz, nx, dz, dx, vel, zs, xs, nsweep, eps):
     return tt
 
 
+def gradient2d_ref(nz, nx, dz, dx, vel, zs, xs, nsweep, eps):
+    """solver2D.F90:984-1032 on the reference's solved field (through
+    refh_solver2d_gradient, which reaches it by the reference's own
+    traceRaysToPoints path)."""
+    R.fteik_solver2d_initialize(c_int(nz), c_int(nx), c_double(0.0),
+                                c_double(0.0), c_double(dz), c_double(dx),
+                                c_int(nsweep), c_double(eps), c_double(0.0),
+                                c_int(0), byref(ierr))
+    chk("solver2d_initialize (gradient)")
+    R.fteik_solver2d_setCellVelocityModel64f(c_int(vel.size), c_int(0),
+                                             D(vel), byref(ierr))
+    R.fteik_solver2d_setSources64f(c_int(1), D(np.array([zs])),
+                                   D(np.array([xs])), byref(ierr))
+    R.fteik_solver2d_solveSourceLSM(c_int(1), byref(ierr))
+    chk("solver2d solve (gradient)")
+    gx = np.zeros(nz * nx)
+    gz = np.zeros(nz * nx)
+    e = c_int(0)
+    R.refh_solver2d_gradient(c_int(nz * nx), D(gx), D(gz), byref(e))
+    # the out-of-model destination makes the call itself fail after the
+    # gradient is computed
+    assert e.value == 1
+    R.fteik_solver2d_free()
+    return gx, gz
+
+
 def solve2d_ref(nz, nx, dz, dx, vel, zs, xs, nsweep, eps, zr, xr):
     R.fteik_solver2d_initialize(c_int(nz), c_int(nx), c_double(0.0),
                                 c_double(0.0), c_double(dz), c_double(dx),
@@ -344,6 +370,11 @@ out["t2d_b_tt"] = tt3
 out["t2d_b_rec"] = np.stack([zr3, xr3])
 out["t2d_b_ttr"] = ttr3
 out["t2d_b_ttcell_xz"] = ttc3
+# the finite-difference gradient of the same two solved fields
+out["t2d_a_gradx"], out["t2d_a_gradz"] = gradient2d_ref(
+    nz2, nx2, dz2, dx2, vel2, zs2, xs2, 2, 5.0)
+out["t2d_b_gradx"], out["t2d_b_gradz"] = gradient2d_ref(
+    nz3, nx3, h3, h3, vel3, zs3, xs3, 3, 4.0)
 # the FSM driver on the same two configurations
 out["t2d_a_fsm_tt"] = solve2d_fsm_ref(nz2, nx2, dz2, dx2, vel2, zs2, xs2, 2,
                                       5.0)

@@ -204,3 +204,64 @@ def test_batch_autotune():
                              4.0)
         np.testing.assert_array_equal(_field(L, k, nz * nx * ny), ref)
     L.fteik_batch3d_free()
+
+
+def test_station_tables_driver_matches_host_tables():
+    """locate_setTravelTimeFieldsFromStations3d (the reciprocal
+    station-as-source driver, fteik.c:186-235): every station's field on
+    the 3D solver's grid and model equals the oracle's single-source solve,
+    and the tables it hands to the locator give the same locations as
+    tables uploaded from the host."""
+    from tests.gpu_helpers import GPU3D
+    nz, nx, ny, h = 21, 24, 22, 100.0
+    vel = hetero(nz, nx, ny, h, 97)
+    g = GPU3D(nz, nx, ny, h, nsweep=2, eps=5.0)
+    g.set_cells(vel)
+    st = np.array([[0.0, 3.3, 4.4], [0.0, 17.9, 2.1], [0.0, 9.5, 19.6]]) * h
+    zs, xs, ys = (np.ascontiguousarray(st[:, i]) for i in range(3))
+    ngrd = nz * nx * ny
+    nev = 8
+    L = lib()
+    ierr = c_int(1)
+    refs = [oracle3d(vel, nz, nx, ny, h, zs[k], xs[k], ys[k], 2, 5.0)[0]
+            for k in range(3)]
+    rng = np.random.default_rng(9)
+    cells = rng.integers(0, ngrd, nev)
+    picks = np.array([[r[c] + 4.0 for r in refs] for c in cells])
+    o2t = np.array([2, 3, 4], dtype=np.int32)
+    w = np.ones(3)
+    results = []
+    for source in ("stations", "host"):
+        L.locate_initialize(nev, 4, ngrd, byref(ierr))
+        assert ierr.value == 0
+        L.locate_setReferenceCompatibility(0, byref(ierr))
+        if source == "stations":
+            L.locate_setTravelTimeFieldsFromStations3d(3, _d(zs), _d(xs),
+                                                       _d(ys), 2, 5.0, 0, 2,
+                                                       byref(ierr))
+            assert ierr.value == 0
+            for k in range(3):
+                np.testing.assert_array_equal(_field(L, k, ngrd), refs[k])
+        else:
+            for k in range(3):
+                L.locate_setTravelTimeField64f(ngrd, k + 2, _d(refs[k]),
+                                               byref(ierr))
+                assert ierr.value == 0
+        one = np.ascontiguousarray(refs[0])
+        L.locate_setTravelTimeField64f(ngrd, 1, _d(one), byref(ierr))
+        for e in range(nev):
+            p = np.ascontiguousarray(picks[e])
+            L.locate_setObservation64f(e + 1, 3, False, 0.0, _i(o2t), _d(p),
+                                       _d(w), byref(ierr))
+        ev = np.arange(1, nev + 1, dtype=np.int32)
+        out = (np.zeros(nev, dtype=np.int32), np.zeros(nev), np.zeros(nev))
+        L.locate_locateEvents(nev, _i(ev), _i(out[0]), _d(out[1]),
+                              _d(out[2]), byref(ierr))
+        assert ierr.value == 0
+        results.append(out)
+        L.locate_setReferenceCompatibility(1, byref(ierr))
+        L.locate_finalize()
+    for a, b in zip(*results):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(results[0][0], cells + 1)
+    L.fteik_batch3d_free()
