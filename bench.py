@@ -293,6 +293,71 @@ def solver2d_section(L, nsrc=256, cpu_sources=4):
     return out
 
 
+def c1_section(L, args, reps=20):
+    """The reference's CPU-sized case: 101^3, h = 10 m, v(z) = 3000 + 0.5 z,
+    source at the centre, 20 seeded surface receivers, 2 iterations, FP64
+    dense (every update performed).  Parity: the field equals the oracle's
+    bit for bit; CPU baseline: the oracle's OpenMP solve of the same case."""
+    n, h, nsweep = 101, 10.0, 2
+    zc = (np.arange(n - 1) + 0.5) * h
+    vel = np.ascontiguousarray(np.tile(3000.0 + 0.5 * zc, (n - 1) * (n - 1)))
+    zs = xs = ys = 50 * h
+    rng = np.random.default_rng(101)
+    xr = rng.uniform(0.0, (n - 1) * h, 20)
+    yr = rng.uniform(0.0, (n - 1) * h, 20)
+    zr = np.zeros(20)
+    dp = ctypes.POINTER(c_double)
+    ierr = c_int(1)
+    L.fteik_solver3d_initialize(n, n, n, 0.0, 0.0, 0.0, h, h, h, nsweep, 5.0,
+                                0.0, 0, byref(ierr))
+    assert ierr.value == 0, "C1 initialize"
+    L.fteik_solver3d_setTileSkipping(0, byref(ierr))
+    L.fteik_solver3d_setVelocityModel64f(vel.size, vel.ctypes.data_as(dp),
+                                         byref(ierr))
+    z, x, y = (np.array([v]) for v in (zs, xs, ys))
+    L.fteik_solver3d_setSources64f(1, z.ctypes.data_as(dp),
+                                   x.ctypes.data_as(dp), y.ctypes.data_as(dp),
+                                   byref(ierr))
+    L.fteik_solver3d_setReceivers64f(20, zr.ctypes.data_as(dp),
+                                     xr.ctypes.data_as(dp),
+                                     yr.ctypes.data_as(dp), byref(ierr))
+    assert ierr.value == 0, "C1 set-up"
+    L.fteik_solver3d_solveSourceLSM(1, byref(ierr))          # warm-up
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.fteik_solver3d_solveSourceLSM(1, byref(ierr))
+    hip_sync()
+    sec = (time.perf_counter() - t0) / reps
+    assert ierr.value == 0, "C1 solve"
+    ms, upd, corr = c_double(), c_double(), c_int()
+    L.fteik_solver3d_getLastSolveInfo(byref(ms), byref(corr), byref(upd),
+                                      byref(ierr))
+    tt = np.zeros(n ** 3)
+    L.fteik_solver3d_getTravelTimeField64f(tt.size, 0, tt.ctypes.data_as(dp),
+                                           byref(ierr))
+    out = {"workload": "101^3, h=10 m, v=3000+0.5z, source at the centre, "
+                       "20 surface receivers, nsweep=2, FP64 dense",
+           "ms_per_solve": sec * 1e3, "value": upd.value / sec / 1e9,
+           "unit": "Gupdates/s", "updates_per_solve": upd.value}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        slow = orc.cell_slowness(vel, n, n, n)
+        src = orc.source_locate((n, n, n, h, h, h, 0.0, 0.0, 0.0), zs, xs, ys,
+                                slow)
+        t0 = time.perf_counter()
+        ref = orc.solve3d(slow, n, n, n, h, h, h, src, nsweep, 5.0,
+                          schedule=1)
+        tcpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": upd.value / tcpu / 1e9,
+                               "unit": "Gupdates/s", "cores": cpu_cores(),
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "one complete solve", "seconds": tcpu}
+        out["gpu_equals_oracle"] = bool(np.array_equal(ref, tt))
+    L.fteik_solver3d_free()
+    return out
+
+
 def locator_section(L, args, nev=10000, cpu_events=16):
     """Full grid search over 48 FP32 station tables built by the batched
     solver and handed to the locator on the device."""
@@ -394,6 +459,10 @@ def locator_section(L, args, nev=10000, cpu_events=16):
     return out
 
 
+# parameters of the 3 -> 256 -> 256 -> 256 -> 256 -> 48 surrogate MLP
+SUR_PARAMS = (3 * 256 + 256) + 3 * (256 * 256 + 256) + (256 * 48 + 48)
+
+
 def surrogate_section(L, stations, grid, nev, args, full_opt, full_s):
     """Learned locator: MLP 3 -> 256^4 -> 48 trained on 2M (cell, times)
     pairs from the locator's tables (seed 13), then Gauss-Newton proposals
@@ -441,6 +510,9 @@ def surrogate_section(L, stations, grid, nev, args, full_opt, full_s):
                         "steps, exact 4x4x4 cube scoring",
             "train_steps": args.sur_steps, "train_batch": 8192,
             "train_seconds": ttrain, "final_loss": loss.value,
+            # 2 (forward) + 4 (backward) flops per parameter and row
+            "train_TFLOPs": 6.0 * SUR_PARAMS * 8192 * args.sur_steps
+                            / ttrain / 1e12,
             "value": nev / tsur, "unit": "events/s", "seconds": tsur,
             "speedup_vs_full_search": full_s / tsur,
             "agreement_with_full_search": float(np.mean(opt == full_opt)),
@@ -782,6 +854,9 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.no_extra:
+        out["c1_101"] = c1_section(L, args)
+        print("bench: 101^3 %.3f ms/solve" % out["c1_101"]["ms_per_solve"],
+              file=sys.stderr, flush=True)
         out["batch"] = batch_section(L, args)
         print("bench: batch %.1f Gupd/s" % out["batch"]["value"],
               file=sys.stderr, flush=True)

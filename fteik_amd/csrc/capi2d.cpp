@@ -21,6 +21,8 @@ struct Solver2D {
     long ttb_cap = 0;
     int *d_ri = nullptr;
     double *d_rd = nullptr, *d_ttr = nullptr;
+    double *d_ttrb = nullptr;        // receiver times of a batch of sources
+    long ttrb_cap = 0;
     double *d_sa = nullptr;          // zsa, xsa per source
     int *d_si = nullptr;             // zsi, xsi per source
     double *d_gx = nullptr, *d_gz = nullptr;   // gradient, on demand
@@ -33,6 +35,7 @@ void free2d()
     dfree(g2.d_slow); dfree(g2.d_tt); dfree(g2.d_scr); dfree(g2.d_ttb);
     dfree(g2.d_ri); dfree(g2.d_rd); dfree(g2.d_ttr);
     dfree(g2.d_sa); dfree(g2.d_si); dfree(g2.d_gx); dfree(g2.d_gz);
+    dfree(g2.d_ttrb);
     g2 = Solver2D();
 }
 
@@ -272,14 +275,27 @@ static bool solve2d_range(int first, int n, const char *who)
                         g2.d_sa + n, g2.d_si, g2.d_si + n, g_dev.stream))
         return false;
     if (g2.nrec > 0) {
-        for (int q = 0; q < n; q++) {
-            if (fteik_k_receivers(g2.d_ttr, fields + (long) q*ngrd, g2.d_ri,
-                                  g2.d_rd, g2.nrec, g2.nz, g2.nx, g2.dz,
-                                  g2.dx, 1.0, 0, g_dev.stream) ||
-                !d2h(g2.ttimesRec.data() + (size_t) (first + q)*g2.nrec,
-                     g2.d_ttr, (size_t) g2.nrec*sizeof(double)))
+        // every source's receiver times in one launch and one copy
+        const long need = (long) n*g2.nrec;
+        if (g2.ttrb_cap < need) {
+            dfree(g2.d_ttrb);
+            g2.ttrb_cap = 0;
+            if (!dalloc(&g2.d_ttrb, (size_t) need)) {
+                msg("%s: out of GPU memory", who);
                 return false;
+            }
+            g2.ttrb_cap = need;
         }
+        for (int q = 0; q < n; q += 65535) {
+            const int m = std::min(65535, n - q);
+            if (fteik_k_receivers_many(g2.d_ttrb + (long) q*g2.nrec,
+                                       fields + (long) q*ngrd, m, ngrd,
+                                       g2.d_ri, g2.d_rd, g2.nrec, g2.nz,
+                                       g2.nx, g2.dz, g2.dx, 1.0, 0,
+                                       g_dev.stream)) return false;
+        }
+        if (!d2h(g2.ttimesRec.data() + (size_t) first*g2.nrec, g2.d_ttrb,
+                 (size_t) need*sizeof(double))) return false;
     }
     if (n > 1 &&
         hipMemcpyAsync(g2.d_tt, fields + (long) (n - 1)*ngrd,

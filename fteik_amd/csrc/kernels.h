@@ -125,6 +125,12 @@ int fteik_k_field_cells(double* out, const double* tt, int nz, int nx,
 int fteik_k_receivers(double* ttr, const double* tt, const int* ri,
                       const double* rd, int nrec, int nz, int nx, double dz,
                       double dx, double dy, int is3d, hipStream_t stream);
+// receiver times of nfield fields fstride doubles apart, field q's to
+// ttr + q*nrec (nfield <= 65535)
+int fteik_k_receivers_many(double* ttr, const double* tt, int nfield,
+                           long fstride, const int* ri, const double* rd,
+                           int nrec, int nz, int nx, double dz, double dx,
+                           double dy, int is3d, hipStream_t stream);
 int fteik_k_d2f(float* out, const double* in, long n, hipStream_t stream);
 int fteik_k_fill_u64(unsigned long long* a, long n, unsigned long long v,
                      hipStream_t stream);

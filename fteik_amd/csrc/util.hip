@@ -161,15 +161,20 @@ __global__ void k_cells(double* __restrict__ out,
 }
 
 // receiver.F90:246-306: trilinear (3D) / bilinear (2D) interpolation from
-// the base node ri (1-based z, x, y) and offsets rd (meters).
+// the base node ri (1-based z, x, y) and offsets rd (meters).  blockIdx.y
+// selects one of several fields fstride apart; its nrec times go to
+// ttr + blockIdx.y*nrec.
 __global__ void k_receivers(double* __restrict__ ttr,
                             const double* __restrict__ tt,
                             const int* __restrict__ ri,
                             const double* __restrict__ rd, int nrec, int nz,
-                            int nx, double dz, double dx, double dy, int is3d)
+                            int nx, double dz, double dx, double dy, int is3d,
+                            long fstride)
 {
     const int r = blockIdx.x*blockDim.x + threadIdx.x;
     if (r >= nrec) return;
+    tt += (long) blockIdx.y*fstride;
+    ttr += (long) blockIdx.y*nrec;
     const long zi = ri[3*r], xi = ri[3*r + 1], yi = ri[3*r + 2];
     const long nzx = (long) nz*nx;
     if (is3d) {
@@ -430,9 +435,20 @@ int fteik_k_receivers(double* ttr, const double* tt, const int* ri,
                       const double* rd, int nrec, int nz, int nx, double dz,
                       double dx, double dy, int is3d, hipStream_t s)
 {
-    if (nrec < 1) return 0;
-    hipLaunchKernelGGL(k_receivers, dim3((nrec + TPB - 1)/TPB), dim3(TPB), 0,
-                       s, ttr, tt, ri, rd, nrec, nz, nx, dz, dx, dy, is3d);
+    return fteik_k_receivers_many(ttr, tt, 1, 0, ri, rd, nrec, nz, nx, dz,
+                                  dx, dy, is3d, s);
+}
+
+int fteik_k_receivers_many(double* ttr, const double* tt, int nfield,
+                           long fstride, const int* ri, const double* rd,
+                           int nrec, int nz, int nx, double dz, double dx,
+                           double dy, int is3d, hipStream_t s)
+{
+    if (nrec < 1 || nfield < 1) return 0;
+    if (nfield > 65535) return 1;
+    hipLaunchKernelGGL(k_receivers, dim3((nrec + TPB - 1)/TPB, nfield),
+                       dim3(TPB), 0, s, ttr, tt, ri, rd, nrec, nz, nx, dz, dx,
+                       dy, is3d, fstride);
     return ok();
 }
 
