@@ -275,6 +275,12 @@ void fteik_solver3d_initialize(int nz, int nx, int ny, double z0, double x0,
         *ierr = 1;
         return;
     }
+    if (!(dz <= kMaxSpacing && dx <= kMaxSpacing && dy <= kMaxSpacing)) {
+        msg("fteik_solver3d_initialize: grid spacing must be finite and at "
+            "most %g", kMaxSpacing);
+        *ierr = 1;
+        return;
+    }
     g3.nz = nz; g3.nx = nx; g3.ny = ny;
     g3.z0 = z0; g3.x0 = x0; g3.y0 = y0;
     g3.dz = dz; g3.dx = dx; g3.dy = dy;
@@ -1042,6 +1048,25 @@ void fteik_amd_evalLocalSolver3d(int n, const double *in, const int *node,
               d2h(outInit, d_o2, (size_t) n*sizeof(double));
     dfree(d_in); dfree(d_node); dfree(d_o1); dfree(d_o2);
     if (!ok) *ierr = 1;
+}
+
+void fteik_amd_sqrtStencil64f(int n, const double *x, double *y, int *ierr)
+{
+    // The iteration stencil's square root of its 2D operators, evaluated on
+    // the GPU for n caller-given arguments (exact for normal x >= 2^-767).
+    static const char *who = "fteik_amd_sqrtStencil64f";
+    *ierr = 0;
+    if (n < 1 || !x || !y) { msg("%s: n must be positive", who); *ierr = 1;
+                             return; }
+    if (!device_ok(who)) { *ierr = 1; return; }
+    double *d_x = nullptr, *d_y = nullptr;
+    bool ok = dalloc(&d_x, (size_t) n) && dalloc(&d_y, (size_t) n) &&
+              h2d(d_x, x, (size_t) n*sizeof(double)) &&
+              fteik_k_sqrt_pos(d_x, d_y, n, g_dev.stream) == 0 &&
+              d2h(y, d_y, (size_t) n*sizeof(double));
+    dfree(d_x);
+    dfree(d_y);
+    if (!ok) { msg("%s: GPU error", who); *ierr = 1; }
 }
 
 void fteik_amd_timeStencilChain3d(int precision, int nsteps,

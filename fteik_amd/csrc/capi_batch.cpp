@@ -298,6 +298,11 @@ static bool batch_set_grid(const char *who, int igrid, int nz, int nx,
         msg("%s: grid %d: need n >= 3 and positive spacing", who, igrid);
         return false;
     }
+    if (!(dz <= kMaxSpacing && dx <= kMaxSpacing && dy <= kMaxSpacing)) {
+        msg("%s: grid %d: spacing must be finite and at most %g", who, igrid,
+            kMaxSpacing);
+        return false;
+    }
     BGrid &x = gb.g[(size_t) (igrid - 1)];
     dfree(x.d_slow); dfree(x.d_tt); dfree(x.d_slow32); dfree(x.d_tt32);
     dfree(x.d_cst); dfree(x.d_cst32);

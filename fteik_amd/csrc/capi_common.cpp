@@ -1,6 +1,7 @@
 // capi_common.cpp -- device set-up and teardown, host-device copies, and the
 // host bookkeeping shared by the solvers: source and receiver grid
 // positions, velocity-model upload, the model geometry.
+#include <cmath>
 #include "capi_internal.h"
 
 namespace fteik_capi {
@@ -79,6 +80,16 @@ double vmin(const double *v, long n)
 {
     double m = v[0];
     for (long i = 1; i < n; i++) m = (v[i] < m) ? v[i] : m;
+    return m;
+}
+
+double vmax(const double *v, long n)
+{
+    double m = v[0];
+    for (long i = 0; i < n; i++) {
+        if (v[i] != v[i]) return HUGE_VAL;       // NaN
+        m = (v[i] > m) ? v[i] : m;
+    }
     return m;
 }
 
@@ -196,6 +207,11 @@ bool set_model(const char *who, bool is3d, int nz, int nx, int ny, int kind,
     if (n < 1) { msg("%s: no points in model %d", who, n); return false; }
     if (vmin(vel, n) <= 0.0) {
         msg("%s: All velocities must be positive", who);
+        return false;
+    }
+    if (!(vmax(vel, n) <= kMaxVelocity)) {
+        msg("%s: velocities must be finite and at most %g m/s", who,
+            kMaxVelocity);
         return false;
     }
     double *d_vel = nullptr;

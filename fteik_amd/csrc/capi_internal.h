@@ -58,6 +58,14 @@ bool h2d(void *d, const void *h, size_t bytes);
 bool d2h(void *h, const void *d, size_t bytes);
 bool sync_ok();
 double vmin(const double *v, long n);
+double vmax(const double *v, long n);     // +inf when an entry is NaN
+
+// Largest accepted velocity and grid spacing.  Together they keep every
+// argument of the iteration stencil's 2D square roots, wherever its gate is
+// open (at least about 4/(v h)^2), a normal number, the range on which the
+// stencil's square root is exact (stencils.h, sqrt_pos).
+constexpr double kMaxVelocity = 1e30;
+constexpr double kMaxSpacing = 1e30;
 
 // Grid position of a source (source.F90:136-190).
 struct SrcInfo {

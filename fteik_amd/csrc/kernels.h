@@ -175,6 +175,7 @@ int fteik_k_time_stencil_chain(const void* cst, int nsteps, int nwg,
 int fteik_k_eval3d(const double* in, const int* node, const void* cst,
                    int n, double* out_noinit, double* out_init,
                    hipStream_t stream);
+int fteik_k_sqrt_pos(const double* x, double* y, long n, hipStream_t stream);
 }
 
 extern "C" {

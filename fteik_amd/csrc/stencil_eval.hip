@@ -61,7 +61,24 @@ __global__ void k_stencil_chain(const fteik::Consts<double>* __restrict__ cd,
     out[blockIdx.x*blockDim.x + threadIdx.x] = t;
 }
 
+// The stencil's shortened square root on caller-given arguments.
+__global__ void k_sqrt_pos(const double* __restrict__ x,
+                           double* __restrict__ y, long n)
+{
+    const long r = (long) blockIdx.x*blockDim.x + threadIdx.x;
+    if (r < n) y[r] = fteik::sqrt_pos(x[r]);
+}
+
 } // namespace
+
+extern "C" int fteik_k_sqrt_pos(const double* x, double* y, long n,
+                                hipStream_t s)
+{
+    if (n < 1) return 0;
+    hipLaunchKernelGGL(k_sqrt_pos, dim3((unsigned) ((n + 255)/256)),
+                       dim3(256), 0, s, x, y, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 // Times one dependent stencil chain per lane of nwg workgroups of one wave.
 extern "C" int fteik_k_time_stencil_chain(const void* cst, int nsteps,

@@ -45,6 +45,56 @@ __device__ __forceinline__ float rmax(float a, float b) { return __builtin_fmaxf
 __device__ __forceinline__ double rsqrt_exact(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float rsqrt_exact(float x) { return __builtin_sqrtf(x); }
 
+// Minimum / maximum of operands read from memory.  The compiler precedes a
+// minnum of a loaded value with a canonicalizing max (it cannot rule out a
+// signalling NaN); the operands here are never NaN, so the hardware
+// instruction alone returns the same value.
+__device__ __forceinline__ double lmin(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double lmax(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float lmin(float a, float b)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float lmax(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Correctly rounded square root of a positive normal x >= 2^-767 (the
+// arguments of the 2D operators of the iteration stencil wherever their
+// gates are open): the reciprocal square-root estimate refined by
+// Newton-Raphson steps in fused multiply-adds (Markstein's method), the
+// sequence the compiler's general square root runs on that range, without
+// its rescaling of tiny arguments and its zero/infinity selection.
+__device__ __forceinline__ double sqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x*y;
+    double h = y*0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+__device__ __forceinline__ float sqrt_pos(float x) { return __builtin_sqrtf(x); }
+
 // localSolver.F90:775-792
 template <typename R>
 __device__ __forceinline__ R tana3d(int i, int j, int k, const Consts<R>& c)
@@ -112,7 +162,7 @@ __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
         const R tab2 = tab*tab;
         const R four_sref2 = R(4)*sref2;
         t1_2d = ((tb*c.dz2i + ta*c.dx2i)
-              + rsqrt_exact(four_sref2*c.dz2i_p_dx2i - c.dz2i_dx2i*tab2))
+              + sqrt_pos(four_sref2*c.dz2i_p_dx2i - c.dz2i_dx2i*tab2))
               *c.dz2i_p_dx2i_inv;
     }
     const bool g1 = temtv < c.dz*s.s4 && -temtv < c.dx*s.s4;
@@ -127,7 +177,7 @@ __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
         const R tab2 = tab*tab;
         const R four_sref2 = R(4)*sref2;
         t2_2d = ((ta*c.dz2i + tb*c.dy2i)
-              + rsqrt_exact(four_sref2*c.dz2i_p_dy2i - c.dz2i_dy2i*tab2))
+              + sqrt_pos(four_sref2*c.dz2i_p_dy2i - c.dz2i_dy2i*tab2))
               *c.dz2i_p_dy2i_inv;
     }
     const bool g2 = tvmtn < c.dy*s.s5 && -tvmtn < c.dz*s.s5;
@@ -142,7 +192,7 @@ __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
         const R tab2 = tab*tab;
         const R four_sref2 = R(4)*sref2;
         t3_2d = ((ta*c.dx2i + tb*c.dy2i)
-              + rsqrt_exact(four_sref2*c.dx2i_p_dy2i - c.dx2i_dy2i*tab2))
+              + sqrt_pos(four_sref2*c.dx2i_p_dy2i - c.dx2i_dy2i*tab2))
               *c.dx2i_p_dy2i_inv;
     }
     const bool g3 = temtn < c.dy*s.s6 && -temtn < c.dx*s.s6;
@@ -168,9 +218,9 @@ __device__ __forceinline__ R noinit3d(const Consts<R>& c, const Nbr<R>& t,
     t12min = rmin(t12min, g1 ? t1_2d : huge);
     t12min = rmin(t12min, g2 ? t2_2d : huge);
     t12min = rmin(t12min, g3 ? t3_2d : huge);
-    const R tmax = rmax(rmax(t.tv, t.te), t.tn);
+    const R tmax = lmax(lmax(t.tv, t.te), t.tn);
     const R t3d = (t12min > tmax && t2 >= t3) ? t3d_v : huge;
-    return rmin(tupd_in, rmin(t3d, t12min));
+    return lmin(tupd_in, rmin(t3d, t12min));
 }
 
 // localSolver.F90:538-751 for one node.  (i, j, k) are 1-based, sgnt* are

@@ -235,13 +235,15 @@ template <typename R>
 __device__ __forceinline__ void slowness7_ud(const R (&zu)[4], const R (&zd)[4],
                                              Slw<R>& s)
 {
-    // zu/zd hold (x,y) = (u,u), (u,d), (d,u), (d,d)
-    s.s1 = rmin(rmin(rmin(zu[0], zu[1]), zu[2]), zu[3]);       // plane i1
-    s.s2 = rmin(rmin(rmin(zu[0], zd[0]), zu[1]), zd[1]);       // plane j1
-    s.s3 = rmin(rmin(rmin(zu[0], zu[2]), zd[0]), zd[2]);       // plane k1
-    s.s4 = rmin(zu[0], zu[1]);                                 // line i1,j1
-    s.s5 = rmin(zu[0], zu[2]);                                 // line i1,k1
-    s.s6 = rmin(zu[0], zd[0]);                                 // line j1,k1
+    // zu/zd hold (x,y) = (u,u), (u,d), (d,u), (d,d); lmin: the cell values
+    // come from memory and are never NaN (a minimum is exact, so the order
+    // of the reductions does not matter)
+    s.s4 = lmin(zu[0], zu[1]);                                 // line i1,j1
+    s.s5 = lmin(zu[0], zu[2]);                                 // line i1,k1
+    s.s6 = lmin(zu[0], zd[0]);                                 // line j1,k1
+    s.s1 = lmin(lmin(s.s4, zu[2]), zu[3]);                     // plane i1
+    s.s2 = lmin(lmin(s.s6, zu[1]), zd[1]);                     // plane j1
+    s.s3 = lmin(lmin(s.s5, zd[0]), zd[2]);                     // plane k1
     s.s7 = zu[0];                                              // cell
 }
 
@@ -582,6 +584,13 @@ __device__ __forceinline__ void mb_get_batch(unsigned long long* mb,
 __device__ __forceinline__ int lds_load(int* p)
 {
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Relaxed read of a step counter; the caller branches on the value before
+// touching the data it guards, and LDS operations of one wave execute in
+// order, so a compiler fence after the wait is all the ordering needed.
+__device__ __forceinline__ int lds_load_rx(int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Mailbox sweep of one tile: BX*BY compute threads (one column each), one
@@ -1034,6 +1043,9 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             __builtin_amdgcn_s_sleep(8);
         }
     }
+    // cached step counters (s_hready, s_wdone[wv -+ 1], s_sdone)
+    int c_h = tlo - 3, c_wm = (wv > 0) ? tlo - 1 : 1 << 30,
+        c_wp = (wv < NW - 1) ? tlo - 1 : 1 << 30, c_s = SW ? tlo - 1 : 1 << 30;
     if (!INIT) prefetch(tlo);
     for (int tb = tlo; tb <= thi; tb += HB) {
         if (INIT) prefetch(tb);
@@ -1052,23 +1064,31 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             if (t > thi) break;                       // workgroup-uniform
             // halo of steps < t, the wave above (w - 1) past step t - 1, and
             // the readers of the ring slot this step overwrites (wave w + 1,
-            // the store wave) past step t - RING + 3
-            unsigned long long spins = 0;
-            while (lds_load(&s_hready) < t ||
-                   (wv > 0 && lds_load(&s_wdone[wv - 1]) < t - 1) ||
-                   (wv < NW - 1 &&
-                    lds_load(&s_wdone[wv + 1]) < t - RING + 3) ||
-                   (SW && lds_load(&s_sdone) < t - RING + 3)) {
-                if (lds_load(&s_abort) || ++spins > kSpinLimit) {
-                    atomicExch(a.err, 1);
-                    __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                    return;
+            // the store wave) past step t - RING + 3.  The counters only
+            // grow: values read earlier that already allow this step are not
+            // read again.
+            if (c_h < t || c_wm < t - 1 || c_wp < t - RING + 3 ||
+                c_s < t - RING + 3) {
+                unsigned long long spins = 0;
+                for (;;) {
+                    c_h = lds_load_rx(&s_hready);
+                    if (wv > 0) c_wm = lds_load_rx(&s_wdone[wv - 1]);
+                    if (wv < NW - 1) c_wp = lds_load_rx(&s_wdone[wv + 1]);
+                    if (SW) c_s = lds_load_rx(&s_sdone);
+                    if (c_h >= t && c_wm >= t - 1 && c_wp >= t - RING + 3 &&
+                        c_s >= t - RING + 3) break;
+                    if (lds_load(&s_abort) || ++spins > kSpinLimit) {
+                        atomicExch(a.err, 1);
+                        __hip_atomic_store(&s_abort, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        return;
+                    }
+                    // a waiting wave yields its issue slots to the waves of
+                    // other tiles sharing the SIMD
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                // a waiting wave yields its issue slots to the waves of
-                // other tiles sharing the SIMD
-                __builtin_amdgcn_s_sleep(1);
             }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             const int fi = t - fj - fk;
             if (colE && fi >= ez0 && fi <= a.fz1) {
                 R v = tc[q];
@@ -1111,10 +1131,14 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                 }
                 ring[t & (RING - 1)][me] = v;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            // LDS operations of one wave execute in order: the counter store
+            // cannot overtake the ring store, so only the compiler needs to
+            // be kept from reordering them
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             if ((threadIdx.x & 63) == 0)
-                __hip_atomic_store(&s_wdone[wv], t, __ATOMIC_RELEASE,
+                __hip_atomic_store(&s_wdone[wv], t, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
         }
     }
     if (a.maxdelta && dmax > R(0)) {

@@ -460,6 +460,12 @@ void fteik_amd_timeSweeps3d(int precision, int nrep, double *msPerLaunch,
  * constants of a solved or prepared source. */
 void fteik_amd_timeStencilChain3d(int precision, int nsteps,
                                   double *usPerStep, int *ierr);
+/* EXTENSION (diagnostic): y = the square root the FP64 iteration stencil
+ * uses for its 2D operators, evaluated on the GPU for n arguments x.  It
+ * equals the correctly rounded sqrt for normal x >= 2^-767, the range of
+ * every such argument wherever its operator's gate is open (velocities and
+ * grid spacings are bounded at 1e30 to keep it so). */
+void fteik_amd_sqrtStencil64f(int n, const double *x, double *y, int *ierr);
 /* EXTENSION: library build string. */
 const char *fteik_amd_version(void);
 /* EXTENSION: releases every solver's device state, then the library's HIP

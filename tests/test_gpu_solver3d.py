@@ -282,6 +282,15 @@ def test_error_paths():
     bad[17] = -1.0
     L.fteik_solver3d_setVelocityModel64f(bad.size, _d(bad), byref(ierr))
     assert ierr.value == 1
+    # velocities above the supported range, and NaN
+    for v in (2e30, np.nan):
+        bad = np.full(11 * 10 * 9, 3000.0)
+        bad[5] = v
+        L.fteik_solver3d_setVelocityModel64f(bad.size, _d(bad), byref(ierr))
+        assert ierr.value == 1
+    ok = np.full(11 * 10 * 9, 1e30)
+    L.fteik_solver3d_setVelocityModel64f(ok.size, _d(ok), byref(ierr))
+    assert ierr.value == 0
     # bad schedule / precision values
     L.fteik_solver3d_setSchedule(7, byref(ierr))
     assert ierr.value == 1
@@ -291,6 +300,40 @@ def test_error_paths():
     z = np.array([-5.0])
     L.fteik_solver3d_setSources64f(1, _d(z), _d(z), _d(z), byref(ierr))
     assert ierr.value == 1
+
+
+def test_stencil_sqrt_is_correctly_rounded():
+    """The iteration stencil's square root (used by its 2D operators)
+    equals IEEE sqrt on its stated range: log-uniform arguments over
+    [2^-767, 2^1023], exact squares, their floating-point neighbours (where
+    rounding is hardest) and the range ends."""
+    L = lib()
+    rng = np.random.default_rng(17)
+    x = [np.exp2(rng.uniform(-767.0, 1023.0, 200000))]
+    r = rng.uniform(1e-100, 1e100, 20000)
+    sq = r * r
+    x += [sq, np.nextafter(sq, 0.0), np.nextafter(sq, np.inf),
+          rng.integers(1, 2**26, 20000).astype(np.float64) ** 2]
+    x.append(np.array([2.0 ** -767, np.finfo(np.float64).max, 1.0, 4.0,
+                       2.0, 0.5]))
+    x = np.ascontiguousarray(np.concatenate(x))
+    x = x[np.isfinite(x) & (x >= 2.0 ** -767)]
+    y = np.zeros_like(x)
+    ierr = c_int(1)
+    L.fteik_amd_sqrtStencil64f(x.size, _d(x), _d(y), byref(ierr))
+    assert ierr.value == 0
+    np.testing.assert_array_equal(y, np.sqrt(x))
+
+
+def test_grid_spacing_limit():
+    L = lib()
+    ierr = c_int(0)
+    L.fteik_solver3d_initialize(9, 8, 7, 0.0, 0.0, 0.0, 10.0, 2e30, 10.0, 2,
+                                5.0, 0.0, 0, byref(ierr))
+    assert ierr.value == 1
+    L.fteik_solver3d_initialize(9, 8, 7, 0.0, 0.0, 0.0, 10.0, 10.0, 10.0, 2,
+                                5.0, 0.0, 0, byref(ierr))
+    assert ierr.value == 0
 
 
 def test_graph_levels_match_oracle():
