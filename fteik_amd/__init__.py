@@ -149,6 +149,7 @@ def load_library(path=None):
         "fteik_amd_timeSweeps3d": [i, i, _dp, _ip],
         "fteik_amd_timeStencilChain3d": [i, i, _dp, _ip],
         "fteik_amd_sqrtStencil64f": [i, _dp, _dp, _ip],
+        "fteik_solver2d_setSchedule": [i, _ip],
         "fteik_batch3d_initialize": [i, _ip],
         "fteik_batch3d_free": [],
         "fteik_batch3d_setGrid": [i, i, i, i, d, d, d, d, d, d, _dp, d, d, d,

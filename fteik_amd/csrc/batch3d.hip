@@ -66,7 +66,8 @@ void batch_sweep_kernel(const BatchGrid* __restrict__ grids,
 // lower-numbered ones and in-order dispatch keeps the lowest unfinished one
 // resident (the launch is larger than the GPU).  args[g]: grid g's sweep
 // arguments for this launch; each grid has its own mailbox.
-template <typename R, bool INIT, int BX, int BY, int HB, int MODE, int KB>
+template <typename R, bool INIT, int BX, int BY, int HB, int MODE, int KB,
+          bool ST>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
 void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
@@ -81,7 +82,7 @@ void batch_mb_kernel(const SweepArgs<R>* __restrict__ args,
 #else   // the host pass only parses the kernel
     const SweepArgs<R> a = args[map[2*blockIdx.x]];
 #endif
-    pencil_mb_body<R, INIT, BX, BY, HB, MODE, false, false, KB>(
+    pencil_mb_body<R, INIT, BX, BY, HB, MODE, false, false, KB, ST>(
         a, dyn_of(a), map[2*blockIdx.x + 1]);
 }
 
@@ -127,15 +128,15 @@ static const void* batch_mb_fn(int tile)
 {
 #define BM(n) case n: return (const void*) batch_mb_kernel<R, INIT, \
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode, \
-        kTiles[n].kb>;
+        kTiles[n].kb, kTiles[n].st != 0>;
     switch (tile) {
         BM(7) BM(8) BM(9) BM(10) BM(11) BM(12) BM(13)
-        BM(14) BM(15) BM(16) BM(17)
+        BM(14) BM(15) BM(16) BM(17) BM(18)
         default: return nullptr;
     }
 #undef BM
 }
-static_assert(kTiles[7].mode == 1 && kTiles[13].mode == 2 && kNumTiles == 18,
+static_assert(kTiles[7].mode == 1 && kTiles[13].mode == 2 && kNumTiles == 19,
               "batch_mb_fn lists the mailbox tiles");
 
 template <typename R, bool INIT>

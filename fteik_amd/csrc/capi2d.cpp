@@ -22,6 +22,10 @@ struct Solver2D {
     int *d_ri = nullptr;
     double *d_rd = nullptr, *d_ttr = nullptr;
     double *d_ttrb = nullptr;        // receiver times of a batch of sources
+    int *d_err = nullptr;            // wait-limit flag of the pipelined sweeps
+    double *d_scr2 = nullptr;        // their per-source scratch lines
+    int scr2_cap = 0;
+    int schedule = 0;                // 0 pipelined columns, 1 level by level
     long ttrb_cap = 0;
     double *d_sa = nullptr;          // zsa, xsa per source
     int *d_si = nullptr;             // zsi, xsi per source
@@ -35,7 +39,7 @@ void free2d()
     dfree(g2.d_slow); dfree(g2.d_tt); dfree(g2.d_scr); dfree(g2.d_ttb);
     dfree(g2.d_ri); dfree(g2.d_rd); dfree(g2.d_ttr);
     dfree(g2.d_sa); dfree(g2.d_si); dfree(g2.d_gx); dfree(g2.d_gz);
-    dfree(g2.d_ttrb);
+    dfree(g2.d_ttrb); dfree(g2.d_err); dfree(g2.d_scr2);
     g2 = Solver2D();
 }
 
@@ -49,6 +53,16 @@ void fteik_solver2d_free(void)
     model_free();
 }
 void fteik_solver2d_setVerbosity(int verbose) { g2.verbose = verbose; }
+void fteik_solver2d_setSchedule(int schedule, int *ierr)
+{
+    *ierr = 0;
+    if (schedule != 0 && schedule != 1) {
+        msg("fteik_solver2d_setSchedule: schedule must be 0 or 1");
+        *ierr = 1;
+        return;
+    }
+    g2.schedule = schedule;
+}
 void fteik_solver2d_setConvergenceTolerance(double convTol)
 {
     g2.convTol = convTol;
@@ -270,10 +284,30 @@ static bool solve2d_range(int first, int n, const char *who)
         !h2d(g2.d_sa, sa.data(), sa.size()*sizeof(double)) ||
         !h2d(g2.d_si, si.data(), si.size()*sizeof(int))) return false;
     double *fields = (n > 1) ? g2.d_ttb : g2.d_tt;
-    if (fteik_k_solve2d(fields, g2.d_slow, g2.nz, g2.nx, g2.dz, g2.dx, g2.eps,
+    if (!g2.d_err && !dalloc(&g2.d_err, 1)) return false;
+    if (g2.scr2_cap < n) {
+        dfree(g2.d_scr2);
+        g2.scr2_cap = 0;
+        if (!dalloc(&g2.d_scr2, (size_t) n*kSolve2dScratch)) {
+            msg("%s: out of GPU memory", who);
+            return false;
+        }
+        g2.scr2_cap = n;
+    }
+    if (hipMemsetAsync(g2.d_err, 0, sizeof(int), g_dev.stream) != hipSuccess ||
+        fteik_k_solve2d(fields, g2.d_slow, g2.nz, g2.nx, g2.dz, g2.dx, g2.eps,
                         0.0, 0.0, 0, 0, g2.nsweep, n, (int) ngrd, g2.d_sa,
-                        g2.d_sa + n, g2.d_si, g2.d_si + n, g_dev.stream))
+                        g2.d_sa + n, g2.d_si, g2.d_si + n, g2.schedule,
+                        g2.d_err, g2.d_scr2, g_dev.stream))
         return false;
+    {
+        int e = 0;
+        if (!d2h(&e, g2.d_err, sizeof(int))) return false;
+        if (e) {
+            msg("%s: a sweep's wait limit was exceeded", who);
+            return false;
+        }
+    }
     if (g2.nrec > 0) {
         // every source's receiver times in one launch and one copy
         const long need = (long) n*g2.nrec;

@@ -153,7 +153,10 @@ int fteik_k_solve2d(double* tt, const double* slow, int nz, int nx,
                     double dz, double dx, double eps, double zsa, double xsa,
                     int zsi, int xsi, int nsweep, int nsrc_batch,
                     int ld_tt, const double* zsa_v, const double* xsa_v,
-                    const int* zsi_v, const int* xsi_v, hipStream_t stream);
+                    const int* zsi_v, const int* xsi_v, int schedule,
+                    int* err, double* scratch, hipStream_t stream);
+// doubles of scratch per source the pipelined 2D sweeps need
+constexpr int kSolve2dScratch = 512;
 
 // locate.hip
 int fteik_k_tables_d2f(float* tables, const double* tt, int ngrd,

@@ -4,9 +4,11 @@
 // reference, :825-867) and nsweep x 4 Gauss-Seidel sweeps over the levels
 // ix+iz (:897-925).  Every sweep is evaluated here level by level -- a
 // topological order of the same dependency graph, so the results equal the
-// reference's bit for bit -- by one workgroup per source: threads own x
-// columns, one barrier per level.  Several sources (solveLSM) run as
-// independent workgroups of one launch.
+// reference's bit for bit -- by one workgroup per source.  Two schedules:
+// solve2d_pipe_kernel (default) pipelines one column per lane without
+// barriers; solve2d_kernel (level by level, one barrier per level) is the
+// cross-check and serves grids wider than 513 nodes.  Several sources
+// (solveLSM) run as independent workgroups of one launch.
 #include <hip/hip_runtime.h>
 #include "stencils.h"
 #include "kernels.h"
@@ -14,6 +16,7 @@
 namespace fteik {
 
 constexpr int T2 = 512;
+constexpr unsigned long long kSpinLimit2D = 1ull << 24;
 
 // localSolver.F90:160-193
 __device__ __forceinline__ void signs2d(int sweep, int& sgntz, int& sgntx)
@@ -110,6 +113,240 @@ void solve2d_kernel(double* __restrict__ ttall, const double* __restrict__ slow,
     }
 }
 
+// ------------------------------------------------------------ pipelined
+// The same sweeps without a barrier per level: one workgroup per source, one
+// lane per flipped x column (nx - 1 <= 512 columns, up to 8 waves), each
+// lane walking down its column one node per step (step L: node fi = L - fj
+// of column fj), so a step's nodes are independent (one level).  A lane
+// keeps its column's node above (fi - 1) in a register; the left column's
+// value of the step before comes from lane - 1 by a wave shift, and the one
+// of two steps before is the shifted value the lane received at the last
+// step.  Lane 0 of wave w > 0 takes lane 63 of wave w - 1 from a ring in LDS
+// after that wave's step counter has passed the step; wave 0's lane 0 reads
+// the frozen column left of the box.  Own old times and slownesses are
+// loaded H2D steps ahead.  The order of the node updates is a topological
+// order of each sweep's dependency graph, as in the level kernel, so the
+// results are the same bit for bit.
+constexpr int R2D = 16;      // ring slots: wave w may lead wave w + 1 by that
+constexpr int T2P = 512;     // largest workgroup (8 waves)
+
+__device__ __forceinline__ double wave_shr1(double x)
+{
+    // lane i gets lane i - 1's value (lane 0 keeps its own)
+    const long long b = __double_as_longlong(x);
+    int lo = (int) b, hi = (int) (b >> 32);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+
+// TMAX threads at most (TMAX/64 waves); H2D steps per prefetch batch
+template <int TMAX, int H2D>
+__global__ __launch_bounds__(TMAX)
+void solve2d_pipe_kernel(double* __restrict__ ttall,
+                         const double* __restrict__ slow, int nz, int nx,
+                         double dz, double dx, double eps, int nsweep,
+                         long ld_tt, const double* __restrict__ zsa_v,
+                         const double* __restrict__ xsa_v,
+                         const int* __restrict__ zsi_v,
+                         const int* __restrict__ xsi_v, int* err,
+                         double* __restrict__ scratch)
+{
+    const int src = blockIdx.x;
+    double* tt = ttall + (long) src*ld_tt;
+    const long ngrd = (long) nz*nx;
+    __shared__ Consts<double> cs;
+    __shared__ double ring[TMAX/64][R2D];
+    __shared__ int s_done[TMAX/64];
+    __shared__ int s_fail;
+    if (threadIdx.x == 0) {
+        // localSolver.F90:299-345
+        Consts<double> c;
+        c.zsa = zsa_v[src]; c.xsa = xsa_v[src]; c.ysa = 1.0;
+        c.zsi = zsi_v[src]; c.xsi = xsi_v[src]; c.ysi = 1;
+        c.szero = slow[(long) (c.xsi - 1)*(nz - 1) + c.zsi - 1];
+        c.szero2 = c.szero*c.szero;
+        c.epsSolver = eps;
+        c.dz = dz; c.dx = dx; c.dy = 1.0;
+        const double dz2 = dz*dz, dx2 = dx*dx;
+        c.dzi = 1.0/dz; c.dxi = 1.0/dx; c.dyi = 1.0;
+        c.dz2i = 1.0/dz2; c.dx2i = 1.0/dx2; c.dy2i = 1.0;
+        c.dz2i_dx2i = c.dz2i*c.dx2i;
+        c.dz2i_p_dx2i = c.dz2i + c.dx2i;
+        c.dz2i_p_dx2i_inv = 1.0/(c.dz2i + c.dx2i);
+        cs = c;
+    }
+    __syncthreads();
+    const Consts<double> c = cs;
+    for (long g = threadIdx.x; g < ngrd; g += blockDim.x) tt[g] = kHuge;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int i = c.zsi + (threadIdx.x & 1), j = c.xsi + (threadIdx.x >> 1);
+        tt[(long) (j - 1)*nz + i - 1] = tana2d(i, j, c);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nwv = blockDim.x >> 6;
+    const int zsi = c.zsi, xsi = c.xsi;
+    const int nsw = 4 + 4*nsweep;
+    bool fail = false;
+    if (threadIdx.x == 0) s_fail = 0;
+    for (int q = 0; q < nsw; q++) {
+        const bool linit = q < 4;
+        const int sweep = (q & 3) + 1;
+        int sgntz, sgntx;
+        signs2d(sweep, sgntz, sgntx);
+        int z0, z1, x0, x1;
+        if (sgntz > 0) { z0 = linit ? max(2, zsi) : 2; z1 = nz; }
+        else           { z0 = 1; z1 = linit ? zsi + 1 : nz - 1; }
+        if (sgntx > 0) { x0 = linit ? max(2, xsi) : 2; x1 = nx; }
+        else           { x0 = 1; x1 = linit ? xsi + 1 : nx - 1; }
+        const int fz0 = (sgntz > 0) ? z0 - 1 : nz - z1;
+        const int fz1 = (sgntz > 0) ? z1 - 1 : nz - z0;
+        const int fx0 = (sgntx > 0) ? x0 - 1 : nx - x1;
+        const int fx1 = (sgntx > 0) ? x1 - 1 : nx - x0;
+        if (fz0 > fz1 || fx0 > fx1) continue;            // uniform
+        // waves with columns in the box, their first and last steps (one
+        // step early, for the left column's value of two steps before)
+        const int wbox = (fx1 - fx0)/64 + 1;
+        const int Lf = fz0 + fx0 + 64*wv - 1;
+        const int Ll = fz1 + min(fx0 + 64*wv + 63, fx1);
+        if (lane == 0 && wv < nwv) s_done[wv] = Lf - 1;
+        __syncthreads();
+        if (s_fail) break;                                // uniform
+        if (wv < wbox) {
+            const int fj = fx0 + 64*wv + lane;
+            const bool colA = fj <= fx1;
+            const int cfj = min(fj, fx1);
+            const int ix = (sgntx > 0) ? cfj + 1 : nx - cfj;
+            double* const col = tt + (long) (ix - 1)*nz;
+            // the frozen column left of the box (wave 0, lane 0)
+            const double* const lcol = tt + (long) (ix - sgntx - 1)*nz;
+            const int xa = max(ix - 1, 1), xb = min(ix, nx - 1);
+            const int ixc = ix - (sgntx > 0 ? 1 : 0);
+            const double* const sa = slow + (long) (xa - 1)*(nz - 1);
+            const double* const sb = slow + (long) (xb - 1)*(nz - 1);
+            const double* const sc = slow + (long) (ixc - 1)*(nz - 1);
+            auto zoff = [&](int fi) -> int {        // 0-based node in column
+                fi = min(max(fi, fz0 - 1), fz1);
+                return (sgntz > 0) ? fi : nz - 1 - fi;
+            };
+            // step s's own old time, the left frozen value and the
+            // slownesses (solver2D.F90:491-502): a[0..5] = t, tleft,
+            // slow(xa, izc), slow(xb, izc), slow(ixc, max(iz-1,1)),
+            // slow(ixc, min(iz,nz-1))
+            double nb[H2D][6];
+            auto load = [&](int s0) {
+                #pragma unroll
+                for (int k = 0; k < H2D; k++) {
+                    const int fi = s0 + k - fj;
+                    const int zn = zoff(fi);
+                    const int iz = zn + 1;
+                    const int izc = min(max(iz - (sgntz > 0 ? 1 : 0), 1),
+                                        nz - 1);
+                    nb[k][0] = col[zn];
+                    nb[k][1] = lcol[zn];       // used by wave 0, lane 0
+                    nb[k][2] = sa[izc - 1];
+                    nb[k][3] = sb[izc - 1];
+                    nb[k][4] = sc[max(iz - 1, 1) - 1];
+                    nb[k][5] = sc[min(iz, nz - 1) - 1];
+                }
+            };
+            double P = col[zoff(fz0 - 1)];          // node above the box
+            double tep = 0.0;
+            int cl = Lf - 2, cr = -(1 << 30);       // cached counters
+            load(Lf);
+            for (int L0 = Lf; L0 <= Ll; L0 += H2D) {
+                double cur[H2D][6];
+                #pragma unroll
+                for (int k = 0; k < H2D; k++)
+                    #pragma unroll
+                    for (int m = 0; m < 6; m++) cur[k][m] = nb[k][m];
+                if (L0 + H2D <= Ll) load(L0 + H2D);
+                #pragma unroll
+                for (int k = 0; k < H2D; k++) {
+                    const int L = L0 + k;
+                    if (L > Ll) break;                   // wave-uniform
+                    const int fi = L - fj;
+                    double te = wave_shr1(P);
+                    if (wv == 0) {
+                        if (lane == 0) te = cur[k][1];
+                    } else if (L - 1 <= fz1 + fx0 + 64*wv - 1) {
+                        // lane 63 of the wave before, after its step L - 1
+                        unsigned long long spins = 0;
+                        while (cl < L - 1) {
+                            cl = __hip_atomic_load(&s_done[wv - 1],
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (cl >= L - 1) break;
+                            if (++spins > kSpinLimit2D) { fail = true; break; }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        const double rv = ring[wv - 1][L % R2D];
+                        if (lane == 0) te = rv;
+                    }
+                    const double tev = tep;
+                    if (colA && fi >= fz0 && fi <= fz1) {
+                        const int iz = (sgntz > 0) ? fi + 1 : nz - fi;
+                        const double s1 = rmin(cur[k][2], cur[k][3]);
+                        const double s2 = rmin(cur[k][4], cur[k][5]);
+                        const double s3 = cur[k][2 + (ixc == xa ? 0 : 1)];
+                        const double t = cur[k][0];
+                        double tupd;
+                        if (linit)
+                            tupd = init2d(c, sgntz, sgntx, iz, ix, P, te, tev,
+                                          t, s1, s2, s3);
+                        else
+                            tupd = noinit2d(c, P, te, tev, s1, s2, s3);
+                        P = rmin(t, tupd);
+                    }
+                    // one store per step in every lane (lanes off the box
+                    // write their slot of the scratch line), so the compiler
+                    // counts this wave's memory operations exactly and the
+                    // prefetched loads are waited for without these stores
+                    {
+                        const bool act = colA && fi >= fz0 && fi <= fz1;
+                        const int iz = (sgntz > 0) ? fi + 1 : nz - fi;
+                        double* const dst = act ? col + (iz - 1)
+                            : scratch + (long) src*T2P + threadIdx.x;
+                        *dst = P;
+                    }
+                    tep = te;
+                    // lane 63's value for the next wave's step L + 1, once
+                    // that wave is past step L + 1 - R2D (the slot's reader)
+                    if (wv + 1 < wbox) {
+                        unsigned long long spins = 0;
+                        while (cr < L + 1 - R2D) {
+                            cr = __hip_atomic_load(&s_done[wv + 1],
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (cr >= L + 1 - R2D) break;
+                            if (++spins > kSpinLimit2D) { fail = true; break; }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        if (lane == 63) ring[wv][(L + 1) % R2D] = P;
+                    }
+                    // LDS operations of one wave execute in order
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane == 0)
+                        __hip_atomic_store(&s_done[wv], L, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (fail) break;
+                }
+                if (fail) break;
+            }
+        }
+        if (fail && lane == 0) {
+            atomicExch(err, 1);
+            s_fail = 1;
+        }
+        __syncthreads();                  // the sweep's field is complete
+    }
+}
+
 } // namespace fteik
 
 using namespace fteik;
@@ -119,11 +356,21 @@ extern "C" int fteik_k_solve2d(double* tt, const double* slow, int nz, int nx,
                                double xsa, int zsi, int xsi, int nsweep,
                                int nsrc_batch, int ld_tt, const double* zsa_v,
                                const double* xsa_v, const int* zsi_v,
-                               const int* xsi_v, hipStream_t stream)
+                               const int* xsi_v, int schedule, int* err,
+                               double* scratch, hipStream_t stream)
 {
     (void) zsa; (void) xsa; (void) zsi; (void) xsi;
-    hipLaunchKernelGGL(solve2d_kernel, dim3(nsrc_batch), dim3(T2), 0, stream,
-                       tt, slow, nz, nx, dz, dx, eps, nsweep, (long) ld_tt,
-                       zsa_v, xsa_v, zsi_v, xsi_v);
+    // pipelined columns when every flipped column of a sweep box has a lane
+    if (schedule == 0 && nx - 1 <= T2P && err && scratch) {
+        const int threads = 64*((nx - 1 + 63)/64);
+        hipLaunchKernelGGL((solve2d_pipe_kernel<T2P, 4>), dim3(nsrc_batch),
+                           dim3(threads), 0, stream, tt, slow, nz, nx, dz, dx,
+                           eps, nsweep, (long) ld_tt, zsa_v, xsa_v, zsi_v,
+                           xsi_v, err, scratch);
+    } else {
+        hipLaunchKernelGGL(solve2d_kernel, dim3(nsrc_batch), dim3(T2), 0,
+                           stream, tt, slow, nz, nx, dz, dx, eps, nsweep,
+                           (long) ld_tt, zsa_v, xsa_v, zsi_v, xsi_v);
+    }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -97,7 +97,7 @@ void sweep3d_pencil_kernel(SweepArgs<R> a)
 // SIMD so two workgroups share a CU.  SKIP: iteration sweep with tile
 // skipping (a separate instance, so the dense sweep keeps its registers)
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE, bool SKIP,
-          int KB>
+          int KB, bool ST>
 __global__ __launch_bounds__(BX*BY + 64*MODE)
 __attribute__((amdgpu_waves_per_eu(waves_per_simd(HB))))
 void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
@@ -105,10 +105,10 @@ void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
     if (SKIP) {
         // the plan lists the processed tiles; the other workgroups exit
         if ((int) blockIdx.x >= a.plan[0]) return;
-        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB>(
+        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB, ST>(
             a, dyn_of(a), a.plan[1 + blockIdx.x]);
     } else {
-        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB>(
+        pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB, ST>(
             a, dyn_of(a), blockIdx.x);
     }
 }
@@ -271,17 +271,17 @@ static const void* pencil_fn_t(int tile)
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb>;
 #define PM(n) case n: return (const void*) sweep3d_pencil_mb_kernel<R, INIT, \
         kTiles[n].bx, kTiles[n].by, kTiles[n].hb, kTiles[n].mode, SKIP, \
-        kTiles[n].kb>;
+        kTiles[n].kb, kTiles[n].st != 0>;
     switch (tile) {
         PF(0) PF(1) PF(2) PF(3) PF(4) PF(5) PF(6)
         PM(7) PM(8) PM(9) PM(10) PM(11) PM(12) PM(13)
-        PM(14) PM(15) PM(16) PM(17)
+        PM(14) PM(15) PM(16) PM(17) PM(18)
         default: return nullptr;
     }
 #undef PF
 #undef PM
 }
-static_assert(kNumTiles == 18 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
+static_assert(kNumTiles == 19 && kTiles[6].mode == 0 && kTiles[7].mode == 1,
               "pencil_fn_t lists every tile");
 
 static int tile_threads(int tile)
@@ -420,7 +420,7 @@ static const void* chain_fn(int tile)
 }
 static_assert(kTiles[10].mode == 2 && kTiles[11].mode == 2 &&
               kTiles[12].mode == 2 && kTiles[13].mode == 2 &&
-              kTiles[13].kb == 1 && kNumTiles == 18,
+              kTiles[13].kb == 1 && kNumTiles == 19,
               "chain_fn lists the mode-2 tiles with per-step mailbox reads");
 
 // Tiles of the iteration-sweep box (flipped columns 1 .. n - 1).

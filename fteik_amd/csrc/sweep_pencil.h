@@ -42,7 +42,14 @@ namespace fteik {
 //     be posted, then loads the batch's words together, so a tile trails
 //     its upwind neighbour by about kb steps but pays one mailbox round
 //     trip per kb steps instead of one per step.
-struct TileShape { int bx, by, hb, mode, kb = 1; };
+// st: staged loads (8 x 8 store-wave tiles with prefetch depth 4): the
+//     compute wave loads a batch's cell layers and own old times
+//     cooperatively, several lanes per column along z, into LDS, instead of
+//     one cache line per lane and load (see pencil_mb_body).  Fewer memory
+//     requests per step: faster where every CU holds several tiles (a 257^2
+//     sweep plane), slower on small grids, where the extra LDS reads of a
+//     step are not hidden.
+struct TileShape { int bx, by, hb, mode, kb = 1, st = 0; };
 constexpr TileShape kTiles[] = {
     {16, 16, 8, 0}, {16, 16, 4, 0}, {16, 16, 2, 0}, {8, 8, 4, 0},
     {8, 8, 2, 0}, {32, 8, 4, 0}, {16, 8, 4, 0},
@@ -53,6 +60,8 @@ constexpr TileShape kTiles[] = {
     {16, 8, 2, 2},
     // batched mailbox reads
     {8, 8, 4, 2, 4}, {8, 8, 4, 2, 2}, {16, 8, 4, 2, 4}, {8, 8, 4, 2, 8},
+    // staged loads
+    {8, 8, 4, 2, 1, 1},
 };
 // Waves per SIMD the kernels of a tile shape are compiled for.  The
 // hardware places a workgroup's waves on the SIMDs from the first one, so a
@@ -626,7 +635,7 @@ __device__ __forceinline__ int lds_load_rx(int* p)
 // steps it releases.  The store wave publishes the tile's own progress with
 // a release after its stores; field loads and stores are device-coherent.
 template <typename R, bool INIT, int BX, int BY, int HB, int MODE,
-          bool SKIP, bool CHAIN = false, int KB = 1>
+          bool SKIP, bool CHAIN = false, int KB = 1, bool ST = false>
 __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                                                const SweepDyn& d,
                                                const int wg)
@@ -645,6 +654,19 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     constexpr int HCOLS = BX + BY + 1;
     static_assert(NC % 64 == 0 && HCOLS <= 64 && HB + 3 <= RING, "tile");
     __shared__ R ring[RING][(BX + 1)*(BY + 1)];
+    // STAGE (staged-load tiles, iteration sweeps outside chains): the cell
+    // layers of a batch of steps are loaded by the compute wave
+    // cooperatively, 8 lanes per cell column along z, and kept in
+    // cbuf[cell column][layer], the own old times likewise (HB lanes per
+    // column) in tbuf: the 81 cell columns and 64 node columns of the tile
+    // cost a few cache lines per load instruction instead of one per lane.
+    static_assert(!ST || (MODE == 2 && NC == 64 && HB == 4 && KB == 1),
+                  "staged loads: 8 x 8 store-wave tiles, prefetch depth 4");
+    constexpr bool STAGE = ST && !INIT && !CHAIN;
+    constexpr int NSL = (BX + 1)*(BY + 1);        // cell columns of the tile
+    constexpr int NSE = (NSL*8 + 63)/64;          // load rounds of 64 lanes
+    __shared__ R cbuf[STAGE ? NSL*8 : 1];
+    __shared__ R tbuf[STAGE ? NC*HB : 1];         // own old times of a batch
     // s_wdone[w]: last step compute wave w completed (wave w reads only
     // wave w-1's cells and the halo, so waves pipeline instead of stepping
     // together); s_hready: halo released through step s_hready - 1;
@@ -1018,16 +1040,82 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     R tn[HB], cn[HB + 1][4];
     auto prefetch = [&](int tb) {
         const int f0 = tb - fdep;
-        #pragma unroll
-        for (int q = 0; q < HB; q++)
-            tn[q] = CHAIN ? ld_coh(col + node_off(f0 + q))
-                          : col[node_off(f0 + q)];
-        #pragma unroll
-        for (int q = 0; q <= HB; q++) {
-            const int z = layer_off(f0 + q);
-            cn[q][0] = c_uu[z]; cn[q][1] = c_ud[z];
-            cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
+        if (!STAGE) {
+            #pragma unroll
+            for (int q = 0; q < HB; q++)
+                tn[q] = CHAIN ? ld_coh(col + node_off(f0 + q))
+                              : col[node_off(f0 + q)];
         }
+        if (!STAGE) {
+            #pragma unroll
+            for (int q = 0; q <= HB; q++) {
+                const int z = layer_off(f0 + q);
+                cn[q][0] = c_uu[z]; cn[q][1] = c_ud[z];
+                cn[q][2] = c_du[z]; cn[q][3] = c_dd[z];
+            }
+        }
+    };
+    // STAGE: element e = lane + 64 i of a batch's cell loads is layer
+    // m = e % 8 of cell column e / 8 = (xa, yb) (x slot 0..BX, y slot
+    // 0..BY; slot a lies between node columns fj0 + a - 1 and fj0 + a).  A
+    // node column (lj, lk) reads slots (lj, lk), (lj, lk + 1), (lj + 1, lk),
+    // (lj + 1, lk + 1) as cells uu, ud, du, dd (the clamps of
+    // autoCode.f90:407-445 are those of the slot's cell column).  For the
+    // batch of steps tb .. tb + 3, slot (a, b) holds the layers of flipped
+    // nodes f = tb - (fj0 + fk0) - a - b + m, m = 0 .. 7: its four readers
+    // need m = q + (a - lj) + (b - lk) and the next one at step tb + q.
+    const R* sptr[STAGE ? NSE : 1];
+    int sab[STAGE ? NSE : 1];
+    R sv[STAGE ? NSE : 1];
+    int cbase[4] = {0, 0, 0, 0};
+    const int sm = threadIdx.x & 7;
+    const int sD = fj0 + fk0;
+    if constexpr (STAGE) {
+        const long szx = (long) (nz - 1)*(nx - 1);
+        #pragma unroll
+        for (int i = 0; i < NSE; i++) {
+            const int sl = min((int) (threadIdx.x + 64*i) >> 3, NSL - 1);
+            const int sa = sl % (BX + 1), sb = sl / (BX + 1);
+            const int cx = (d.sx > 0) ? min(max(fj0 + sa, 1), nx - 1)
+                                      : min(max(nx - fj0 - sa, 1), nx - 1);
+            const int cy = (d.sy > 0) ? min(max(fk0 + sb, 1), ny - 1)
+                                      : min(max(ny - fk0 - sb, 1), ny - 1);
+            sptr[i] = a.slow + (long) (cy - 1)*szx + (long) (cx - 1)*(nz - 1);
+            sab[i] = sa + sb;
+        }
+        // LDS element of cell uu (slot (lj, lk), delta 0), ud (lj, lk + 1),
+        // du (lj + 1, lk) (delta 1 each) and dd (lj + 1, lk + 1) (delta 2)
+        const int slj = threadIdx.x % BX, slk = threadIdx.x / BX;
+        cbase[0] = (slk*(BX + 1) + slj)*8;
+        cbase[1] = ((slk + 1)*(BX + 1) + slj)*8 + 1;
+        cbase[2] = (slk*(BX + 1) + slj + 1)*8 + 1;
+        cbase[3] = ((slk + 1)*(BX + 1) + slj + 1)*8 + 2;
+    }
+    // STAGE: the batch's own old times, element e = lane + 64 i being step
+    // e % HB of node column e / HB (HB lanes per column along z)
+    constexpr int NTE = STAGE ? NC*HB/64 : 1;
+    const R* tptr[NTE];
+    int tdep[NTE];
+    if constexpr (STAGE) {
+        #pragma unroll
+        for (int i = 0; i < NTE; i++) {
+            const int cix = (int) (threadIdx.x + 64*i) / HB;
+            const int tj = min(max(fj0 + cix % BX, jlo), jhi);
+            const int tk = min(max(fk0 + cix / BX, klo), khi);
+            tptr[i] = a.tt + (long) (unflip(tk, ny, d.sy) - 1)*nz*nx
+                      + (long) (unflip(tj, nx, d.sx) - 1)*nz;
+            tdep[i] = tj + tk;
+        }
+    }
+    R tv_[NTE];
+    auto stage_load = [&](int tb) {
+        #pragma unroll
+        for (int i = 0; i < NSE; i++)
+            sv[i] = sptr[i][layer_off(tb - sD - sab[i] + sm)];
+        #pragma unroll
+        for (int i = 0; i < NTE; i++)
+            tv_[i] = tptr[i][node_off(tb + (((int) threadIdx.x + 64*i) % HB)
+                                      - tdep[i])];
     };
     if (CHAIN) {
         // the halo wave's release of step tlo - 3 covers the loads below
@@ -1047,21 +1135,47 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
     int c_h = tlo - 3, c_wm = (wv > 0) ? tlo - 1 : 1 << 30,
         c_wp = (wv < NW - 1) ? tlo - 1 : 1 << 30, c_s = SW ? tlo - 1 : 1 << 30;
     if (!INIT) prefetch(tlo);
+    if constexpr (STAGE) stage_load(tlo);
     for (int tb = tlo; tb <= thi; tb += HB) {
         if (INIT) prefetch(tb);
         R tc[HB], cc[HB + 1][4];
         #pragma unroll
         for (int q = 0; q < HB; q++) tc[q] = tn[q];
-        #pragma unroll
-        for (int q = 0; q <= HB; q++) {
-            cc[q][0] = cn[q][0]; cc[q][1] = cn[q][1];
-            cc[q][2] = cn[q][2]; cc[q][3] = cn[q][3];
+        if constexpr (STAGE) {
+            // this batch's cell layers into the buffer (the wave's own LDS
+            // accesses stay in order: the last batch's reads are done)
+            #pragma unroll
+            for (int i = 0; i < NSE; i++) {
+                const int e = (int) threadIdx.x + 64*i;
+                if (e < NSL*8) cbuf[e] = sv[i];
+            }
+            #pragma unroll
+            for (int i = 0; i < NTE; i++) tbuf[threadIdx.x + 64*i] = tv_[i];
+            if (tb + HB <= thi) stage_load(tb + HB);
+        } else {
+            #pragma unroll
+            for (int q = 0; q <= HB; q++) {
+                cc[q][0] = cn[q][0]; cc[q][1] = cn[q][1];
+                cc[q][2] = cn[q][2]; cc[q][3] = cn[q][3];
+            }
         }
         if (!INIT && tb + HB <= thi) prefetch(tb + HB);
         #pragma unroll
         for (int q = 0; q < HB; q++) {
             const int t = tb + q;
             if (t > thi) break;                       // workgroup-uniform
+            // STAGE: this step's slowness terms need no other wave: they are
+            // formed before the wait below
+            Slw<R> sls;
+            if constexpr (STAGE) {
+                R zu[4], zd[4];
+                #pragma unroll
+                for (int c4 = 0; c4 < 4; c4++) {
+                    zu[c4] = cbuf[cbase[c4] + q];
+                    zd[c4] = cbuf[cbase[c4] + q + 1];
+                }
+                slowness7_ud<R>(zu, zd, sls);
+            }
             // halo of steps < t, the wave above (w - 1) past step t - 1, and
             // the readers of the ring slot this step overwrites (wave w + 1,
             // the store wave) past step t - RING + 3.  The counters only
@@ -1092,6 +1206,7 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             const int fi = t - fj - fk;
             if (colE && fi >= ez0 && fi <= a.fz1) {
                 R v = tc[q];
+                if constexpr (STAGE) v = tbuf[threadIdx.x*HB + q];
                 if (colB && fi >= a.fz0) {
                     const int s1 = (t - 1) & (RING - 1),
                               s2 = (t - 2) & (RING - 1),
@@ -1105,7 +1220,11 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
                     nb.tnv  = ring[s2][cS];
                     nb.tnve = ring[s3][cSW];
                     Slw<R> sl;
-                    slowness7_ud<R>(cc[q], cc[q + 1], sl);
+                    if constexpr (STAGE) {
+                        sl = sls;
+                    } else {
+                        slowness7_ud<R>(cc[q], cc[q + 1], sl);
+                    }
                     R vn;
                     if (INIT) {
                         const int i = unflip(fi, nz, d.sz);

@@ -196,6 +196,11 @@ void fteik_solver2d_setNumberOfSweeps(int nsweep, int *ierr);
 void fteik_solver2d_setSphereToCartEpsilon(double eps, int *ierr);
 void fteik_solver2d_setVerbosity(int verbose);
 void fteik_solver2d_setConvergenceTolerance(double convTol);
+/* EXTENSION: 0 = pipelined columns, one lane per x column walking down z,
+ * no barrier per level (default; grids with nx <= 513, otherwise the
+ * level schedule runs); 1 = level by level with a workgroup barrier per
+ * level (diagnostic).  Both give the same results bit for bit. */
+void fteik_solver2d_setSchedule(int schedule, int *ierr);
 /* solver2D.F90:283, :310: rays to the last solved source (as the 3D
  * functions; the reference's 2D tracer, rays.F90:449-669, stops after its
  * first ray). */

@@ -52,8 +52,8 @@ def _field(L, g, n):
     return tt
 
 
-@pytest.mark.parametrize("tile", [0, 3, 6, 7, 8, 9, 10, 11, 12, 13,
-                                  (3, 12), (0, 10)])
+@pytest.mark.parametrize("tile", [0, 3, 6, 7, 8, 9, 10, 11, 12, 13, 18,
+                                  (3, 12), (0, 10), (3, 18)])
 def test_batch_matches_oracle(tile):
     print("case: batch tile", tile, flush=True)
     models = [hetero(nz, nx, ny, h, 40 + k)
@@ -71,7 +71,7 @@ def test_batch_matches_oracle(tile):
     L.fteik_batch3d_free()
 
 
-@pytest.mark.parametrize("tile", [6, 13, 9])
+@pytest.mark.parametrize("tile", [6, 13, 9, 18])
 def test_large_batch_matches_oracle(tile):
     """48 grids of 65^3 (four sources, each twelve times): more workgroups
     per sweep than the GPU holds at once, so later grids' tiles wait for
@@ -160,7 +160,7 @@ def test_batch_error_paths():
     L.fteik_batch3d_free()
 
 
-@pytest.mark.parametrize("tile", [3, 6, 11, 13])
+@pytest.mark.parametrize("tile", [3, 6, 11, 13, 18])
 def test_batch_mixed_precision_matches_fp64(tile):
     """Mixed mode (FP32 iterations, FP64 corrections to convergence) of a
     batch: every grid is within 1e-6

@@ -140,6 +140,55 @@ def test_solve2d_many_sources_one_launch():
             np.testing.assert_array_equal(g.field(), ref)
 
 
+@pytest.mark.parametrize("nz,nx", [(57, 83), (45, 513), (33, 40),
+                                   (120, 130), (64, 514)])
+def test_solve2d_schedules_match_oracle(nz, nx):
+    """Pipelined columns (one lane per x column, several waves for wide
+    grids; grids wider than 513 nodes fall back to the level schedule) and
+    the level-by-level schedule give the oracle's fields bit for bit, for
+    sources near every corner and in the middle (initialization boxes of
+    every extent)."""
+    h = 10.0
+    rng = np.random.default_rng(nz * 1000 + nx)
+    from tests.golden.gen_golden_models import smooth_noise
+    zc = (np.arange(nz - 1) + 0.5) * h
+    vel = ((1800.0 + 0.9 * zc)[None, :] *
+           (1.0 + 0.1 * smooth_noise(rng, (nx - 1, nz - 1), 3.0))).ravel()
+    zs = np.array([0.2, nz - 2.1, 0.0, nz - 2.5, 0.5 * nz]) * h
+    xs = np.array([0.3, 0.0, nx - 2.2, nx - 2.4, 0.5 * nx]) * h
+    zr = np.zeros(5)
+    xr = np.linspace(0.0, (nx - 1) * h, 5)
+    slow = orc.cell_slowness(vel, nz, nx)
+    grid = (nz, nx, 1, h, h, 1.0, 0.0, 0.0, 0.0)
+    refs = []
+    for k in range(zs.size):
+        src = orc.source_locate(grid, zs[k], xs[k], slow=slow, is3d=False)
+        refs.append(orc.solve2d(slow, nz, nx, h, h, src, 2, 3.0))
+    ierr = c_int(1)
+    for sched in (0, 1):
+        print("case: 2D", nz, nx, "schedule", sched, flush=True)
+        g = GPU2D(nz, nx, h, h, 2, 3.0)
+        g.L.fteik_solver2d_setSchedule(sched, byref(ierr))
+        assert ierr.value == 0
+        g.set_cells(vel)
+        g.set_receivers(zr, xr)
+        for k in range(zs.size):
+            g.set_sources(zs[k], xs[k])
+            g.L.fteik_solver2d_solveSourceLSM(1, byref(ierr))
+            assert ierr.value == 0
+            np.testing.assert_array_equal(g.field(), refs[k])
+        # all sources in one launch: receiver times
+        g.set_sources(zs, xs)
+        g.L.fteik_solver2d_solveLSM(byref(ierr))
+        assert ierr.value == 0
+        ttr = g.receivers()
+        for k in range(zs.size):
+            np.testing.assert_array_equal(
+                ttr[k], orc.receivers(refs[k], grid, zr, xr, is3d=False))
+    g.L.fteik_solver2d_setSchedule(2, byref(ierr))
+    assert ierr.value == 1
+
+
 def test_solve2d_error_paths():
     L = lib()
     g = GPU2D(10, 12, 10.0, 10.0, 2, 3.0)

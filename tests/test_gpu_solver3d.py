@@ -398,7 +398,8 @@ def test_tiles_launched_without_coresidency(skipping):
     store-wave tiles (3 waves each) are not all resident at once: the
     sweeps launch them without cooperation, in tile order.  The field equals
     the oracle's and the cooperative 8 x 8 tiles' bit for bit, with and
-    without tile skipping."""
+    without tile skipping, also with the staged loads (tile 18, the default
+    of such grids)."""
     nz, nx, ny, h = 129, 257, 257, 20.0
     vel = hetero(nz, nx, ny, h, 12)
     zs, xs, ys = 60.3 * h, 100.7 * h, 200.2 * h
@@ -410,7 +411,10 @@ def test_tiles_launched_without_coresidency(skipping):
     ierr = c_int(1)
     L.fteik_solver3d_setSweepOverlap(0, byref(ierr))
     L.fteik_solver3d_setTileSkipping(skipping, byref(ierr))
-    for tile in (12, 9):
+    t, t32, nt = c_int(), c_int(), c_int()
+    L.fteik_solver3d_getTile(byref(t), byref(t32), byref(nt), byref(ierr))
+    assert t.value == 18 and t32.value == 18
+    for tile in (18, 12, 9):
         _announce("large grid tile", tile, "skipping", skipping)
         L.fteik_solver3d_setTile(tile, tile, byref(ierr))
         assert ierr.value == 0
