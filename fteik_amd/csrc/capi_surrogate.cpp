@@ -311,17 +311,18 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
     if (!sur_work(batch)) { msg("%s: out of GPU memory", who); *ierr = 1;
                             return; }
     hipStream_t s = g_dev.stream;
-    const double zero = 0.0;
     bool ok = true;
     double tail = 0.0;
     const int tailN = std::max(1, std::min(steps, 50));
     for (int it = 0; it < steps && ok; it++) {
         const long first = (long) it*batch;
+        // the loss is cleared on the stream (no host synchronization per
+        // step, so the host queues the next steps while the GPU works)
         ok = fteik_k_sur_samples(gl.d_tab, gl.ldgrd, gs.d_geo, gl.ngrd,
                                  gs.seed, pool, first, batch, gs.d_x,
                                  gs.d_tgt, gs.ntf, s) == 0 &&
-             h2d(gs.d_loss, &zero, sizeof(double)) && sur_forward(batch) &&
-             sur_backward(batch);
+             hipMemsetAsync(gs.d_loss, 0, sizeof(double), s) == hipSuccess &&
+             sur_forward(batch) && sur_backward(batch);
         gs.step++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f/(1.0f - powf(b1, (float) gs.step));

@@ -414,6 +414,10 @@ def test_tiles_launched_without_coresidency(skipping):
     t, t32, nt = c_int(), c_int(), c_int()
     L.fteik_solver3d_getTile(byref(t), byref(t32), byref(nt), byref(ierr))
     assert t.value == 18 and t32.value == 18
+    # the FP64 launches below are the non-cooperative ones: a tile plane of
+    # this grid does not fit the GPU at once (library's own residency check)
+    for tile in (18, 12):
+        assert L.fteik_k_pencil_fits(nx, ny, 0, tile) == 0
     for tile in (18, 12, 9):
         _announce("large grid tile", tile, "skipping", skipping)
         L.fteik_solver3d_setTile(tile, tile, byref(ierr))
