@@ -21,10 +21,15 @@ struct Surrogate {
     unsigned short *d_pb = nullptr;    // BF16 copy of all parameters
     // work buffers (grown on demand)
     long cap = 0;
-    float *d_x = nullptr, *d_tgt = nullptr, *d_y = nullptr, *d_dh = nullptr;
+    float *d_x = nullptr, *d_tgt = nullptr, *d_y = nullptr;
     float *d_z[4] = {nullptr, nullptr, nullptr, nullptr};
     unsigned short *d_h[4] = {nullptr, nullptr, nullptr, nullptr};
-    unsigned short *d_dy = nullptr, *d_dz = nullptr;
+    unsigned short *d_dy = nullptr, *d_dza = nullptr, *d_dzb = nullptr;
+    // gradient parts: split-K partial products of the weight gradients
+    // (grown on demand), column sums per 64-row block of the bias
+    // gradients (5 slots), first-layer sums per row chunk
+    float *d_P = nullptr, *d_colp = nullptr, *d_dw1p = nullptr;
+    long capP = 0;
     double *d_loss = nullptr;
     unsigned long long seed = 13;
     // inversion work buffers (grown on demand): event arrays sized by
@@ -83,7 +88,8 @@ void free_surrogate()
 {
     dfree(gs.d_geo); dfree(gs.d_p); dfree(gs.d_g); dfree(gs.d_m); dfree(gs.d_v);
     dfree(gs.d_pb); dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y);
-    dfree(gs.d_dh); dfree(gs.d_dy); dfree(gs.d_dz); dfree(gs.d_loss);
+    dfree(gs.d_dy); dfree(gs.d_dza); dfree(gs.d_dzb); dfree(gs.d_P);
+    dfree(gs.d_colp); dfree(gs.d_dw1p); dfree(gs.d_loss);
     dfree(gs.d_zero);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     free_inv_work();
@@ -91,16 +97,22 @@ void free_surrogate()
     gs = Surrogate();
 }
 
+constexpr int kDw1Chunks = 64;        // row chunks of the first layer's sums
+
+long row_blocks(long rows) { return (rows + 63)/64; }
+
 bool sur_work(long rows)
 {
     if (rows <= gs.cap) return true;
-    dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y); dfree(gs.d_dh);
-    dfree(gs.d_dy); dfree(gs.d_dz);
+    dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y); dfree(gs.d_dy);
+    dfree(gs.d_dza); dfree(gs.d_dzb); dfree(gs.d_colp); dfree(gs.d_dw1p);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     const size_t R = (size_t) rows, H = (size_t) gs.H, T = (size_t) gs.ntf;
     bool ok = dalloc(&gs.d_x, 3*R) && dalloc(&gs.d_tgt, R*T) &&
-              dalloc(&gs.d_y, R*T) && dalloc(&gs.d_dh, R*H) &&
-              dalloc(&gs.d_dy, R*T) && dalloc(&gs.d_dz, R*H);
+              dalloc(&gs.d_y, R*T) && dalloc(&gs.d_dy, R*T) &&
+              dalloc(&gs.d_dza, R*H) && dalloc(&gs.d_dzb, R*H) &&
+              dalloc(&gs.d_colp, 5*(size_t) row_blocks(rows)*H) &&
+              dalloc(&gs.d_dw1p, (size_t) kDw1Chunks*3*H);
     for (int l = 0; l < 4 && ok; l++)
         ok = dalloc(&gs.d_z[l], R*H) && dalloc(&gs.d_h[l], R*H);
     if (!ok) { gs.cap = 0; return false; }
@@ -108,58 +120,89 @@ bool sur_work(long rows)
     return true;
 }
 
-// Forward pass over `rows` rows of d_x into d_y (network output, before
-// the baseline and scale are applied).
-bool sur_forward(long rows)
-{
-    hipStream_t s = g_dev.stream;
-    const int H = gs.H;
-    const unsigned short *pb = gs.d_pb;
-    bool ok = fteik_k_sur_layer1(gs.d_x, gs.d_p + gs.oW1, gs.d_p + gs.ob1,
-                                 gs.d_z[0], gs.d_h[0], rows, H, s) == 0;
-    for (int l = 0; l < 3 && ok; l++) {
-        // z_{l+1} = h_l W^T + b
-        ok = fteik_k_gemm_bf16(gs.d_h[l], H, 1, pb + gs.oW[l], 1, H,
-                               gs.d_z[l + 1], H, (int) rows, H, H,
-                               gs.d_p + gs.ob[l], 0, s) == 0 &&
-             fteik_k_sur_silu(gs.d_z[l + 1], gs.d_h[l + 1], rows*H, s) == 0;
-    }
-    ok = ok && fteik_k_gemm_bf16(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, gs.d_y,
-                                 gs.ntf, (int) rows, gs.ntf, H,
-                                 gs.d_p + gs.ob5, 0, s) == 0;
-    return ok;
-}
-
-// Forward + backward of the mean squared error against d_tgt; parameter
-// gradients into d_g, loss (mean over rows x ntf) added to d_loss.
-bool sur_backward(long rows)
+// Forward pass over `rows` rows of d_x.  Inference (train false): network
+// output (before the baseline and scale) into d_y.  Training: the output
+// layer's GEMM forms dy = d(mean squared error)/dy against d_tgt in its
+// epilogue (with the loss and the output bias gradient's parts) instead.
+bool sur_forward(long rows, bool train)
 {
     hipStream_t s = g_dev.stream;
     const int H = gs.H, T = gs.ntf;
     const unsigned short *pb = gs.d_pb;
-    const float inv = 1.0f/(float) ((double) rows*T);
-    bool ok = fteik_k_sur_mse(gs.d_y, gs.d_tgt, gs.d_dy, rows*T, inv,
-                              gs.d_loss, s) == 0;
-    // output layer: dW5 = dy^T h4, db5 = colsum(dy), dh4 = dy W5
-    ok = ok && fteik_k_gemm_bf16(gs.d_dy, 1, T, gs.d_h[3], H, 1,
-                                 gs.d_g + gs.oW5, H, T, H, (int) rows,
-                                 nullptr, 0, s) == 0 &&
-         fteik_k_sur_colsum(gs.d_dy, rows, T, gs.d_g + gs.ob5, s) == 0 &&
-         fteik_k_gemm_bf16(gs.d_dy, T, 1, pb + gs.oW5, H, 1, gs.d_dh, H,
-                           (int) rows, H, T, nullptr, 0, s) == 0 &&
-         fteik_k_sur_silu_bwd(gs.d_dh, gs.d_z[3], gs.d_dz, rows*H, s) == 0;
-    for (int l = 2; l >= 0 && ok; l--) {
-        // layer l+1 (weights oW[l]) maps h_l -> z_{l+1}; dz holds dz_{l+1}
-        ok = fteik_k_gemm_bf16(gs.d_dz, 1, H, gs.d_h[l], H, 1,
-                               gs.d_g + gs.oW[l], H, H, H, (int) rows,
-                               nullptr, 0, s) == 0 &&
-             fteik_k_sur_colsum(gs.d_dz, rows, H, gs.d_g + gs.ob[l], s) == 0 &&
-             fteik_k_gemm_bf16(gs.d_dz, H, 1, pb + gs.oW[l], H, 1, gs.d_dh,
-                               H, (int) rows, H, H, nullptr, 0, s) == 0 &&
-             fteik_k_sur_silu_bwd(gs.d_dh, gs.d_z[l], gs.d_dz, rows*H, s) == 0;
+    bool ok = fteik_k_sur_layer1(gs.d_x, gs.d_p + gs.oW1, gs.d_p + gs.ob1,
+                                 gs.d_z[0], gs.d_h[0], rows, H, s) == 0;
+    for (int l = 0; l < 3 && ok; l++) {
+        // z_{l+1} = h_l W^T + b, h_{l+1} = silu(z_{l+1})
+        ok = fteik_k_gemm_silu(gs.d_h[l], H, 1, pb + gs.oW[l], 1, H,
+                               (int) rows, H, H, gs.d_p + gs.ob[l],
+                               gs.d_z[l + 1], gs.d_h[l + 1], H, s) == 0;
     }
-    ok = ok && fteik_k_sur_dw1(gs.d_dz, gs.d_x, rows, H, gs.d_g + gs.oW1, s) == 0
-            && fteik_k_sur_colsum(gs.d_dz, rows, H, gs.d_g + gs.ob1, s) == 0;
+    if (!ok) return false;
+    if (!train)
+        return fteik_k_gemm_bf16(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, gs.d_y,
+                                 T, (int) rows, T, H, gs.d_p + gs.ob5, 0,
+                                 s) == 0;
+    const float inv = 1.0f/(float) ((double) rows*T);
+    return fteik_k_gemm_mse(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, (int) rows, T,
+                            H, gs.d_p + gs.ob5, gs.d_tgt, gs.d_dy, T, inv,
+                            gs.d_loss, gs.d_colp + 3*row_blocks(rows)*H,
+                            s) == 0;
+}
+
+// Backward pass after sur_forward(rows, true): the parts of every
+// parameter gradient (split-K weight-gradient products, bias column sums
+// per row block, first-layer sums per row chunk), and their segment
+// table for k_grad_adam in parameter order.
+bool sur_backward(long rows, FteikGradSeg *seg, int *nseg)
+{
+    hipStream_t s = g_dev.stream;
+    const int H = gs.H, T = gs.ntf, M = (int) rows;
+    const unsigned short *pb = gs.d_pb;
+    const long nrb = row_blocks(rows);
+    const int nsH = fteik_k_gemm_parts(H, H, M);
+    const int ns5 = fteik_k_gemm_parts(T, H, M);
+    const long needP = 3L*nsH*H*H + (long) ns5*T*H;
+    if (needP > gs.capP) {
+        dfree(gs.d_P);
+        gs.capP = 0;
+        if (!dalloc(&gs.d_P, (size_t) needP)) return false;
+        gs.capP = needP;
+    }
+    float *P5 = gs.d_P + 3L*nsH*H*H;
+    auto colp = [&](int slot) { return gs.d_colp + slot*nrb*H; };
+    // output layer: dW5 = dy^T h3; dz3 = (dy W5) silu'(z3), whose column
+    // sums are the bias gradient of the layer that made z3
+    bool ok = fteik_k_gemm_partial(gs.d_dy, 1, T, gs.d_h[3], H, 1, T, H, M,
+                                   P5, s) == 0 &&
+              fteik_k_gemm_silu_bwd(gs.d_dy, T, 1, pb + gs.oW5, H, 1, M, H, T,
+                                    gs.d_z[3], gs.d_dza, H, colp(2), s) == 0;
+    unsigned short *cur = gs.d_dza, *nxt = gs.d_dzb;
+    for (int l = 2; l >= 0 && ok; l--) {
+        // layer l+1 (weights oW[l]) maps h_l -> z_{l+1}; cur holds dz_{l+1}
+        ok = fteik_k_gemm_partial(cur, 1, H, gs.d_h[l], H, 1, H, H, M,
+                                  gs.d_P + (long) l*nsH*H*H, s) == 0 &&
+             fteik_k_gemm_silu_bwd(cur, H, 1, pb + gs.oW[l], H, 1, M, H, H,
+                                   gs.d_z[l], nxt, H,
+                                   colp(l >= 1 ? l - 1 : 4), s) == 0;
+        std::swap(cur, nxt);
+    }
+    ok = ok && fteik_k_sur_dw1_parts(cur, gs.d_x, rows, H, kDw1Chunks,
+                                     gs.d_dw1p, s) == 0;
+    int q = 0;
+    auto add = [&](long off, long len, const float *src, int np, long ps) {
+        seg[q].off = off; seg[q].len = len; seg[q].src = src;
+        seg[q].nparts = np; seg[q].pstride = ps;
+        q++;
+    };
+    add(gs.oW1, 3L*H, gs.d_dw1p, kDw1Chunks, 3L*H);
+    add(gs.ob1, H, colp(4), (int) nrb, H);
+    for (int l = 0; l < 3; l++) {
+        add(gs.oW[l], (long) H*H, gs.d_P + (long) l*nsH*H*H, nsH, (long) H*H);
+        add(gs.ob[l], H, colp(l), (int) nrb, H);
+    }
+    add(gs.oW5, (long) T*H, P5, ns5, (long) T*H);
+    add(gs.ob5, T, colp(3), (int) nrb, T);
+    *nseg = q;
     return ok;
 }
 
@@ -266,7 +309,8 @@ void fteik_surrogate_forward(int n, const float *pos, float *out, int *ierr)
 {
     *ierr = 0;
     if (!gs.init || n < 1 || !sur_work(n) ||
-        !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) || !sur_forward(n) ||
+        !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) ||
+        !sur_forward(n, false) ||
         !d2h(out, gs.d_y, (size_t) n*gs.ntf*sizeof(float))) {
         msg("fteik_surrogate_forward: failed");
         *ierr = 1;
@@ -280,11 +324,16 @@ void fteik_surrogate_gradients(int n, const float *pos, const float *tgt,
 {
     *ierr = 0;
     const double zero = 0.0;
+    FteikGradSeg seg[kSurMaxSeg];
+    int nseg = 0;
     if (!gs.init || n < 1 || !sur_work(n) ||
         !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) ||
         !h2d(gs.d_tgt, tgt, (size_t) n*gs.ntf*sizeof(float)) ||
-        !h2d(gs.d_loss, &zero, sizeof(double)) || !sur_forward(n) ||
-        !sur_backward(n) ||
+        !h2d(gs.d_loss, &zero, sizeof(double)) || !sur_forward(n, true) ||
+        !sur_backward(n, seg, &nseg) ||
+        fteik_k_sur_grad_adam(seg, nseg, gs.d_g, gs.d_p, gs.d_m, gs.d_v,
+                              gs.d_pb, gs.np, 0, 0.0f, 0.0f, 0.0f, 0.0f,
+                              0.0f, 0.0f, g_dev.stream) != 0 ||
         !d2h(grad, gs.d_g, (size_t) gs.np*sizeof(float)) ||
         !d2h(loss, gs.d_loss, sizeof(double))) {
         msg("fteik_surrogate_gradients: failed");
@@ -314,22 +363,25 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
     bool ok = true;
     double tail = 0.0;
     const int tailN = std::max(1, std::min(steps, 50));
+    FteikGradSeg seg[kSurMaxSeg];
+    int nseg = 0;
     for (int it = 0; it < steps && ok; it++) {
         const long first = (long) it*batch;
-        // the loss is cleared on the stream (no host synchronization per
-        // step, so the host queues the next steps while the GPU works)
+        // the sample kernel clears the loss on the stream (no host
+        // synchronization per step, so the host queues the next steps
+        // while the GPU works)
         ok = fteik_k_sur_samples(gl.d_tab, gl.ldgrd, gs.d_geo, gl.ngrd,
                                  gs.seed, pool, first, batch, gs.d_x,
-                                 gs.d_tgt, gs.ntf, s) == 0 &&
-             hipMemsetAsync(gs.d_loss, 0, sizeof(double), s) == hipSuccess &&
-             sur_forward(batch) && sur_backward(batch);
+                                 gs.d_tgt, gs.ntf, gs.d_loss, s) == 0 &&
+             sur_forward(batch, true) && sur_backward(batch, seg, &nseg);
         gs.step++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f/(1.0f - powf(b1, (float) gs.step));
         const float c2 = 1.0f/(1.0f - powf(b2, (float) gs.step));
         const float lrt = (float) (lr*0.5*(1.0 + cos(3.14159265358979*it/steps)));
-        ok = ok && fteik_k_sur_adam(gs.d_p, gs.d_g, gs.d_m, gs.d_v, gs.np, lrt,
-                                    b1, b2, 1e-8f, c1, c2, gs.d_pb, s) == 0;
+        ok = ok && fteik_k_sur_grad_adam(seg, nseg, gs.d_g, gs.d_p, gs.d_m,
+                                         gs.d_v, gs.d_pb, gs.np, 1, lrt, b1,
+                                         b2, 1e-8f, c1, c2, s) == 0;
         if (ok && it >= steps - tailN) {
             double l = 0.0;
             ok = d2h(&l, gs.d_loss, sizeof(double));

@@ -204,26 +204,45 @@ int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
                       long sbk, long sbn, float* C, long ldc, int M, int N,
                       int K, const float* bias, int accumulate,
                       hipStream_t stream);
+// training GEMMs with fused epilogues (k <= 256 except the partial form)
+int fteik_k_gemm_parts(int M, int N, int K);
+int fteik_k_gemm_partial(const void* A, long sam, long sak, const void* B,
+                         long sbk, long sbn, int M, int N, int K, float* P,
+                         hipStream_t stream);
+int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
+                      long sbk, long sbn, int M, int N, int K,
+                      const float* bias, float* z, void* h, long ld,
+                      hipStream_t stream);
+int fteik_k_gemm_silu_bwd(const void* A, long sam, long sak, const void* B,
+                          long sbk, long sbn, int M, int N, int K,
+                          const float* z, void* dz, long ld, float* colp,
+                          hipStream_t stream);
+int fteik_k_gemm_mse(const void* A, long sam, long sak, const void* B,
+                     long sbk, long sbn, int M, int N, int K,
+                     const float* bias, const float* tgt, void* dy, long ld,
+                     float inv, double* loss, float* colp, hipStream_t stream);
 int fteik_k_f2bf(const float* a, void* b, long n, hipStream_t stream);
 void fteik_k_sur_release(void);
 int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
                        float* z, void* h, long rows, int H, hipStream_t stream);
-int fteik_k_sur_silu(const float* z, void* h, long n, hipStream_t stream);
-int fteik_k_sur_silu_bwd(const float* dh, const float* z, void* dz, long n,
-                         hipStream_t stream);
-int fteik_k_sur_mse(const float* y, const float* tgt, void* dy, long n,
-                    float inv, double* loss, hipStream_t stream);
-int fteik_k_sur_colsum(const void* d, long rows, int cols, float* out,
-                       hipStream_t stream);
-int fteik_k_sur_dw1(const void* dz, const float* x, long rows, int H,
-                    float* dW1, hipStream_t stream);
-int fteik_k_sur_adam(float* p, const float* g, float* m, float* v, long n,
-                     float lr, float b1, float b2, float eps, float c1,
-                     float c2, void* pb, hipStream_t stream);
+int fteik_k_sur_dw1_parts(const void* dz, const float* x, long rows, int H,
+                          int nchunks, float* out, hipStream_t stream);
+// A range of parameters whose gradient is the in-order sum of nparts parts
+// src[z*pstride + i - off].
+struct FteikGradSeg {
+    long off, len, pstride;
+    const float* src;
+    int nparts;
+};
+constexpr int kSurMaxSeg = 12;
+int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
+                          float* p, float* m, float* v, void* pb, long n,
+                          int adam, float lr, float b1, float b2, float eps,
+                          float c1, float c2, hipStream_t stream);
 int fteik_k_sur_samples(const float* tab, long ldgrd, const void* geo,
                         long ngrd, unsigned long long seed, long pool,
                         long first, long rows, float* x, float* tgt, int ldt,
-                        hipStream_t stream);
+                        double* loss, hipStream_t stream);
 int fteik_k_sur_geo_size(void);
 int fteik_k_inv_start(float* pos, long R, int starts, hipStream_t stream);
 int fteik_k_inv_hidden(const void* hin, long R, const void* W,

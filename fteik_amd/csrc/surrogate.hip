@@ -56,120 +56,310 @@ inline int nblocks(long n)
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
 // ------------------------------------------------------------------ GEMM
-// C[m][n] (+)= sum_k A(m,k) B(k,n) (+ bias[n]), A(m,k) = A[m*sam + k*sak],
-// B(k,n) = B[k*sbk + n*sbn], BF16 operands, FP32 accumulation and output.
-// Workgroup tile 64 x 64, four waves of 32 x 32, K in chunks of 32 staged
-// through LDS as [row][k] so each lane reads its 8 consecutive k of the
-// v_mfma_f32_32x32x16_bf16 operand with one 16-byte LDS read:
+// C[m][n] = sum_k A(m,k) B(k,n), then an epilogue; A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn], BF16 operands, FP32 accumulation.
+// Workgroup tile 64 x 64, four waves of 32 x 32 (v_mfma_f32_32x32x16_bf16):
 //   A/B operand: lane l holds row (l % 32), k = 8*(l / 32) .. +7;
 //   C: lane l, register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
-constexpr int GT = 64, GK = 32, GPAD = 8;
+// A workgroup stages up to GKW = 256 values of k of both operands into LDS
+// as [row][k] at once (all its global loads in flight together, then one
+// barrier) and runs its MFMAs in increasing k, 16 at a time; the network's
+// layers have k <= 256, longer k loops over rounds.  The accumulation order
+// is the same for every staging, so every epilogue sees the same sums.
+constexpr int GT = 64, GK = 32, GKW = 256, LDK = GKW + 8;
 
-// Staging of one operand tile [64 rows][32 k] into LDS as [row][k].  The
-// global reads follow the operand's layout: with k contiguous (sk == 1) a
-// thread reads 8 consecutive k of one row as one 16-byte load; with rows
-// contiguous (sr == 1, a transposed operand) it reads 8 consecutive rows
-// of one k, so a wave's reads are contiguous either way.
-__device__ __forceinline__ void stage_tile(const bf16_t* __restrict__ X,
-                                           long sr, long sk, long r0, int R,
-                                           int k0, int K,
-                                           bf16_t (*Xs)[GK + GPAD], int t)
+// Epilogues:
+//   EP_STORE     C = acc + bias (+ C when accumulating)
+//   EP_SPLIT     P[z][m][n] = acc over the k range of split z
+//   EP_SILU      z = acc + bias into C, silu(z) as BF16 into hout
+//   EP_SILU_BWD  dz = acc * silu'(zin) as BF16 into hout, and the column
+//                sums of those BF16 values over the tile's 64 rows into
+//                colp[blockIdx.y][n] (bias gradient parts)
+//   EP_MSE       y = acc + bias, dy = 2 (y - tgt) inv as BF16 into hout, its
+//                column sums into colp, inv * sum (y - tgt)^2 added to loss
+enum { EP_STORE = 0, EP_SPLIT = 1, EP_SILU = 2, EP_SILU_BWD = 3, EP_MSE = 4 };
+
+struct GemmArgs {
+    const bf16_t* A; long sam, sak;
+    const bf16_t* B; long sbk, sbn;
+    int M, N, K, kper;
+    float* C; long ldc;          // C, zin, hout and tgt share the stride ldc
+    const float* bias;
+    int accumulate;
+    float* P;
+    const float* zin;
+    bf16_t* hout;
+    float* colp;
+    const float* tgt;
+    float inv;
+    double* loss;
+};
+
+// Thread t's 16-byte piece of an operand tile [64 rows][32 k] (rows r0..,
+// k kc..kc+31), zero outside rows [0, R) and k [kc, ke).  The global reads
+// follow the operand's layout: with k contiguous (sk == 1) 8 consecutive k
+// of one row; with rows contiguous (sr == 1, a transposed operand) 8
+// consecutive rows of one k; otherwise element by element.
+__device__ __forceinline__ uint4 tile_fetch(const bf16_t* __restrict__ X,
+                                            long sr, long sk, long r0, int R,
+                                            int kc, int ke, int t)
 {
+    union { uint4 v; bf16_t h[8]; } u;
     if (sk == 1) {
-        const int lr = t >> 2, lk = (t & 3)*8;
-        const long r = r0 + lr;
-        const int k = k0 + lk;
+        const long r = r0 + (t >> 2);
+        const int k = kc + (t & 3)*8;
         const bf16_t* p = X + r*sr + k;
-        if (r < R && k + 7 < K && (sr & 7) == 0 &&
-            (((uintptr_t) p) & 15) == 0) {
-            *(uint4*) &Xs[lr][lk] = *(const uint4*) p;
-        } else {
-            #pragma unroll
-            for (int q = 0; q < 8; q++)
-                Xs[lr][lk + q] = (r < R && k + q < K) ? p[q] : (bf16_t) 0;
-        }
-    } else if (sr == 1) {
-        const int lk = t & 31, lr = (t >> 5)*8;
-        const long r = r0 + lr;
-        const int k = k0 + lk;
-        const bf16_t* p = X + (long) k*sk + r;
-        bf16_t v[8];
-        if (k < K && r + 7 < R && (((uintptr_t) p) & 15) == 0) {
-            const uint4 w = *(const uint4*) p;
-            const bf16_t* wv = (const bf16_t*) &w;
-            #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = wv[q];
-        } else {
-            #pragma unroll
-            for (int q = 0; q < 8; q++)
-                v[q] = (k < K && r + q < R) ? p[q] : (bf16_t) 0;
-        }
+        if (r < R && k + 7 < ke && (((uintptr_t) p) & 15) == 0)
+            return *(const uint4*) p;
         #pragma unroll
-        for (int q = 0; q < 8; q++) Xs[lr + q][lk] = v[q];
+        for (int q = 0; q < 8; q++)
+            u.h[q] = (r < R && k + q < ke) ? p[q] : (bf16_t) 0;
+    } else if (sr == 1) {
+        const long r = r0 + (t >> 5)*8;
+        const int k = kc + (t & 31);
+        const bf16_t* p = X + (long) k*sk + r;
+        if (k < ke && r + 7 < R && (((uintptr_t) p) & 15) == 0)
+            return *(const uint4*) p;
+        #pragma unroll
+        for (int q = 0; q < 8; q++)
+            u.h[q] = (k < ke && r + q < R) ? p[q] : (bf16_t) 0;
     } else {
-        const int lr = t >> 2, lk = (t & 3)*8;
-        const long r = r0 + lr;
+        const long r = r0 + (t >> 2);
         #pragma unroll
         for (int q = 0; q < 8; q++) {
-            const int k = k0 + lk + q;
-            Xs[lr][lk + q] = (r < R && k < K) ? X[r*sr + (long) k*sk]
-                                              : (bf16_t) 0;
+            const int k = kc + (t & 3)*8 + q;
+            u.h[q] = (r < R && k < ke) ? X[r*sr + (long) k*sk] : (bf16_t) 0;
+        }
+    }
+    return u.v;
+}
+
+// Whether an operand's tiles can be loaded branch-free: 1 with k
+// contiguous (rows 16-byte aligned, the k range a multiple of 8), 2 with
+// rows contiguous (k strides 16-byte aligned, R a multiple of 8), 0
+// otherwise (element-checked loads).  Rows past R are read as row R - 1
+// (mode 1) or as the last 8 rows (mode 2): they only reach outputs past
+// the matrix, which are not stored.  Values of k past the range are read
+// at k0 and replaced by zeros.
+__device__ __forceinline__ int full_tile(const bf16_t* X, long sr, long sk,
+                                         int R, int ke)
+{
+    if ((((uintptr_t) X) & 15) != 0) return 0;
+    if (sk == 1 && (sr & 7) == 0 && (ke & 7) == 0) return 1;
+    if (sr == 1 && (sk & 7) == 0 && (R & 7) == 0 && R >= 8) return 2;
+    return 0;
+}
+
+// Thread t's pieces of a round (chunks c = 0..GKW/GK-1 from k0) for a
+// mode full_tile accepted.
+__device__ __forceinline__ void full_round(const bf16_t* __restrict__ X,
+                                           long sr, long sk, long r0, int R,
+                                           int k0, int ke, int t, int mode,
+                                           uint4* v)
+{
+    constexpr int NCH = GKW/GK;
+    if (mode == 1) {
+        const long r = min(r0 + (t >> 2), (long) R - 1);
+        const bf16_t* p = X + r*sr;
+        #pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int k = k0 + c*GK + (t & 3)*8;
+            const bool ok = k < ke;
+            const uint4 u = *(const uint4*) (p + (ok ? k : k0));
+            v[c] = ok ? u : make_uint4(0, 0, 0, 0);
+        }
+    } else {
+        const long r = min(r0 + (t >> 5)*8, (long) R - 8);
+        #pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const int k = k0 + c*GK + (t & 31);
+            const bool ok = k < ke;
+            const uint4 u = *(const uint4*) (X + (long) (ok ? k : k0)*sk + r);
+            v[c] = ok ? u : make_uint4(0, 0, 0, 0);
         }
     }
 }
 
-// One workgroup: the 64 x 64 tile (blockIdx.y, blockIdx.x) of C over the
-// k range of split blockIdx.z (kper values each).  nsplit == 1 writes C
-// (with bias, optionally accumulating); otherwise the partial sums go to
-// P[z][M][N] for k_gemm_reduce.
-__global__ __launch_bounds__(256)
-void k_gemm_bf16(const bf16_t* __restrict__ A, long sam, long sak,
-                 const bf16_t* __restrict__ B, long sbk, long sbn,
-                 float* __restrict__ C, long ldc, int M, int N, int K,
-                 const float* __restrict__ bias, int accumulate, int kper,
-                 float* __restrict__ P)
+// Stores tile_fetch's piece into the LDS tile [row][k] at k offset c*GK.
+__device__ __forceinline__ void tile_put(bf16_t (*Xs)[LDK], long sr, long sk,
+                                         int c, int t, uint4 v)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t As[GT][GK + GPAD];
-    __shared__ __attribute__((aligned(16))) bf16_t Bs[GT][GK + GPAD];
+    if (sk == 1 || sr != 1) {
+        *(uint4*) &Xs[t >> 2][c*GK + (t & 3)*8] = v;
+    } else {
+        union { uint4 v; bf16_t h[8]; } u;
+        u.v = v;
+        #pragma unroll
+        for (int q = 0; q < 8; q++) Xs[(t >> 5)*8 + q][c*GK + (t & 31)] = u.h[q];
+    }
+}
+
+// Tile of this workgroup.  Workgroups are dispatched to the GPU's 8 XCDs
+// in turn (XCD = linear id mod 8), each XCD with its own L2: the tiles are
+// dealt so that XCD q gets the q-th eighth of them in (column tile, row
+// block, split) order, i.e. the column tiles of one row block (which read
+// the same A rows) and the tiles of one split (the same k range) share an
+// L2.
+__device__ __forceinline__ void gemm_tile(int& bx, int& by, int& bz)
+{
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int T = nx*ny*gridDim.z;
+    int L = blockIdx.x + nx*(blockIdx.y + ny*blockIdx.z);
+    if ((T & 7) == 0) L = (L & 7)*(T >> 3) + (L >> 3);
+    bx = L % nx;
+    by = (L / nx) % ny;
+    bz = L / (nx*ny);
+}
+
+// One workgroup: the 64 x 64 tile (by, bx) of C over the k range of split
+// bz (kper values each).
+template <int EP>
+__global__ __launch_bounds__(256)
+void k_gemm(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t As[GT][LDK];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[GT][LDK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
-    const long m0 = (long) blockIdx.y*GT, n0 = (long) blockIdx.x*GT;
-    const int kb = blockIdx.z*kper, ke = min(K, kb + kper);
+    int bx, by, bz;
+    gemm_tile(bx, by, bz);
+    const long m0 = (long) by*GT, n0 = (long) bx*GT;
+    const int kb = bz*g.kper, ke = min(g.K, kb + g.kper);
+    constexpr int NCH = GKW/GK;
     f32x16 acc = {};
-    for (int k0 = kb; k0 < ke; k0 += GK) {
-        stage_tile(A, sam, sak, m0, M, k0, ke, As, t);
-        stage_tile(B, sbn, sbk, n0, N, k0, ke, Bs, t);
-        __syncthreads();
-        #pragma unroll
-        for (int kk = 0; kk < GK; kk += 16) {
-            const bf16x8 a = *(const bf16x8*) &As[wm*32 + (lane & 31)]
-                                                  [kk + 8*(lane >> 5)];
-            const bf16x8 b = *(const bf16x8*) &Bs[wn*32 + (lane & 31)]
-                                                  [kk + 8*(lane >> 5)];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    for (int k0 = kb; k0 < ke; k0 += GKW) {
+        const int nch = (min(GKW, ke - k0) + GK - 1)/GK;
+        uint4 va[NCH], vb[NCH];
+        // operands full_tile accepts load branch-free, every load of the
+        // round in flight at once
+        const int fa = full_tile(g.A, g.sam, g.sak, g.M, ke);
+        const int fb = full_tile(g.B, g.sbn, g.sbk, g.N, ke);
+        if (fa) {
+            full_round(g.A, g.sam, g.sak, m0, g.M, k0, ke, t, fa, va);
+        } else {
+            #pragma unroll
+            for (int c = 0; c < NCH; c++)
+                if (c < nch)
+                    va[c] = tile_fetch(g.A, g.sam, g.sak, m0, g.M, k0 + c*GK,
+                                       ke, t);
         }
+        if (fb) {
+            full_round(g.B, g.sbn, g.sbk, n0, g.N, k0, ke, t, fb, vb);
+        } else {
+            #pragma unroll
+            for (int c = 0; c < NCH; c++)
+                if (c < nch)
+                    vb[c] = tile_fetch(g.B, g.sbn, g.sbk, n0, g.N, k0 + c*GK,
+                                       ke, t);
+        }
+        if (k0 > kb) __syncthreads();      // the last round's reads are done
+        #pragma unroll
+        for (int c = 0; c < NCH; c++)
+            if (c < nch) {
+                tile_put(As, g.sam, g.sak, c, t, va[c]);
+                tile_put(Bs, g.sbn, g.sbk, c, t, vb[c]);
+            }
         __syncthreads();
+        const bf16_t* ar = &As[wm*32 + (lane & 31)][8*(lane >> 5)];
+        const bf16_t* br = &Bs[wn*32 + (lane & 31)][8*(lane >> 5)];
+        if (nch == NCH) {
+            // operands of the next 4 k steps are read while the current 4
+            // MFMAs run
+            bf16x8 a0[4], b0[4];
+            #pragma unroll
+            for (int e = 0; e < 4; e++) {
+                a0[e] = *(const bf16x8*) (ar + 16*e);
+                b0[e] = *(const bf16x8*) (br + 16*e);
+            }
+            __builtin_amdgcn_iglp_opt(0);
+            #pragma unroll
+            for (int q = 0; q < GKW/64; q++) {
+                bf16x8 a1[4], b1[4];
+                if (q + 1 < GKW/64) {
+                    #pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        a1[e] = *(const bf16x8*) (ar + 64*(q + 1) + 16*e);
+                        b1[e] = *(const bf16x8*) (br + 64*(q + 1) + 16*e);
+                    }
+                }
+                #pragma unroll
+                for (int e = 0; e < 4; e++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[e], b0[e],
+                                                                  acc, 0, 0, 0);
+                if (q + 1 < GKW/64) {
+                    #pragma unroll
+                    for (int e = 0; e < 4; e++) { a0[e] = a1[e]; b0[e] = b1[e]; }
+                }
+            }
+        } else {
+            for (int kk = 0; kk < nch*GK; kk += 16)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    *(const bf16x8*) (ar + kk), *(const bf16x8*) (br + kk),
+                    acc, 0, 0, 0);
+        }
     }
     const long n = n0 + wn*32 + (lane & 31);
-    if (n >= N) return;
-    if (P) {
-        float* Pz = P + (long) blockIdx.z*M*N;
+    const bool nok = n < g.N;
+    auto row = [&](int r) -> long {
+        return m0 + wm*32 + 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+    };
+    if (EP == EP_SPLIT) {
+        if (!nok) return;
+        float* Pz = g.P + (long) bz*g.M*g.N;
         #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const long m = m0 + wm*32 + 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
-            if (m < M) Pz[m*N + n] = acc[r];
+            const long m = row(r);
+            if (m < g.M) Pz[m*g.N + n] = acc[r];
         }
-        return;
-    }
-    const float bn = bias ? bias[n] : 0.0f;
-    #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const long m = m0 + wm*32 + 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
-        if (m < M) {
-            float v = acc[r] + bn;
-            if (accumulate) v += C[m*ldc + n];
-            C[m*ldc + n] = v;
+    } else if (EP == EP_STORE || EP == EP_SILU) {
+        if (!nok) return;
+        const float bn = g.bias ? g.bias[n] : 0.0f;
+        #pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const long m = row(r);
+            if (m < g.M) {
+                float v = acc[r] + bn;
+                if (EP == EP_STORE && g.accumulate) v += g.C[m*g.ldc + n];
+                g.C[m*g.ldc + n] = v;
+                if (EP == EP_SILU) g.hout[m*g.ldc + n] = f2bf(silu(v));
+            }
         }
+    } else {
+        __shared__ float csum[2][32];
+        __shared__ double lsum[4];
+        const float bn = (EP == EP_MSE && g.bias && nok) ? g.bias[n] : 0.0f;
+        float s = 0.0f;
+        double l = 0.0;
+        #pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const long m = row(r);
+            if (m < g.M && nok) {
+                const long o = m*g.ldc + n;
+                bf16_t q;
+                if (EP == EP_SILU_BWD) {
+                    q = f2bf((acc[r] + bn)*dsilu(g.zin[o]));
+                } else {
+                    const float d = (acc[r] + bn) - g.tgt[o];
+                    q = f2bf(2.0f*d*g.inv);
+                    l += (double) d*d;
+                }
+                g.hout[o] = q;
+                s += bf2f(q);
+            }
+        }
+        s += __shfl_xor(s, 32);           // lanes l and l + 32: same column
+        if (wm == 1 && lane < 32) csum[wn][lane] = s;
+        if (EP == EP_MSE) {
+            #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+            if (lane == 0) lsum[w] = l;
+        }
+        __syncthreads();
+        if (wm == 0 && lane < 32 && nok)
+            g.colp[(long) by*g.N + n] = s + csum[wn][lane];
+        if (EP == EP_MSE && t == 0)
+            atomicAdd(g.loss, ((lsum[0] + lsum[1]) + (lsum[2] + lsum[3]))
+                              *(double) g.inv);
     }
 }
 
@@ -188,7 +378,41 @@ __global__ void k_gemm_reduce(const float* __restrict__ P, int nsplit, int M,
     }
 }
 
-// workspace of the split-K partial products (grown on demand)
+// Split of k for an M x N GEMM over K: few output tiles over a long k (the
+// weight gradients: 256 x 256 over the batch rows) split k so the GPU's
+// 256 CUs all work; the parts are summed in a fixed order, so the result
+// does not depend on timing.
+int gemm_split(int M, int N, int K, int* kper)
+{
+    const int tiles = ((N + GT - 1)/GT)*((M + GT - 1)/GT);
+    int nsplit = 1;
+    if (tiles < 256 && K >= 2048)
+        nsplit = std::min(std::max(512/tiles, 1), K/256);
+    const int kp = ((K + nsplit - 1)/nsplit + GK - 1)/GK*GK;
+    *kper = kp;
+    return (K + kp - 1)/kp;
+}
+
+template <int EP>
+int launch_gemm(const GemmArgs& g, int nsplit, hipStream_t s)
+{
+    dim3 grid((g.N + GT - 1)/GT, (g.M + GT - 1)/GT, nsplit);
+    hipLaunchKernelGGL(k_gemm<EP>, grid, dim3(256), 0, s, g);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+GemmArgs gemm_args(const void* A, long sam, long sak, const void* B,
+                   long sbk, long sbn, int M, int N, int K)
+{
+    GemmArgs g = {};
+    g.A = (const bf16_t*) A; g.sam = sam; g.sak = sak;
+    g.B = (const bf16_t*) B; g.sbk = sbk; g.sbn = sbn;
+    g.M = M; g.N = N; g.K = K; g.kper = K;
+    return g;
+}
+
+// workspace of the split-K partial products of fteik_k_gemm_bf16 (grown
+// on demand)
 float* g_split = nullptr;
 size_t g_split_cap = 0;
 
@@ -208,96 +432,87 @@ __global__ void k_layer1(const float* __restrict__ x, const float* __restrict__ 
     }
 }
 
-__global__ void k_silu(const float* __restrict__ z, bf16_t* __restrict__ h,
-                       long n)
+// Parts of the first layer's weight gradient: chunk c of `per` rows gives
+// out[c][3 j + d] = sum over its rows r of dz0[r][j] x[r][d].  Thread
+// (u, j) of the H x 4 block sums the u-th quarter of the chunk for hidden
+// unit j (a row's dz0 values are read contiguously); the quarters are
+// added in order.
+__global__ __launch_bounds__(1024)
+void k_dw1_part(const bf16_t* __restrict__ dz, const float* __restrict__ x,
+                long rows, int H, long per, float* __restrict__ out)
 {
-    GRID_STRIDE(i, n) h[i] = f2bf(silu(z[i]));
-}
-
-// dz = dh * silu'(z), to BF16 for the next GEMMs (and FP32 for the bias
-// gradient)
-__global__ void k_silu_bwd(const float* __restrict__ dh,
-                           const float* __restrict__ z,
-                           bf16_t* __restrict__ dz, long n)
-{
-    GRID_STRIDE(i, n) dz[i] = f2bf(dh[i]*dsilu(z[i]));
-}
-
-// Mean squared error over rows x nout; dy = 2 (y - t) / (rows nout).
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ tgt,
-                      bf16_t* __restrict__ dy, long n, float inv,
-                      double* __restrict__ loss)
-{
-    __shared__ double red[TPB];
-    double s = 0.0;
-    GRID_STRIDE(i, n) {
-        const float d = y[i] - tgt[i];
-        dy[i] = f2bf(2.0f*d*inv);
-        s += (double) d*d;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = TPB/2; o > 0; o >>= 1) {
-        if ((int) threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]*(double) inv);
-}
-
-// Column sums of a BF16 [rows][cols] matrix (bias gradients).
-__global__ void k_colsum(const bf16_t* __restrict__ d, long rows, int cols,
-                         float* __restrict__ out)
-{
-    const int c = blockIdx.x;
-    float s = 0.0f;
-    for (long r = threadIdx.x; r < rows; r += blockDim.x)
-        s += bf2f(d[r*cols + c]);
-    __shared__ float red[TPB];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = TPB/2; o > 0; o >>= 1) {
-        if ((int) threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = red[0];
-}
-
-// dW1[j][d] = sum_r dz1[r][j] x[r][d]
-__global__ void k_dw1(const bf16_t* __restrict__ dz, const float* __restrict__ x,
-                      long rows, int H, float* __restrict__ dW1)
-{
-    const int j = blockIdx.x;
+    __shared__ float red[3][4][256];
+    const int j = threadIdx.x, u = threadIdx.y;
+    const long c0 = (long) blockIdx.x*per, q = (per + 3)/4;
+    const long r0 = min(rows, c0 + u*q), r1 = min(rows, min(c0 + per, r0 + q));
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-    for (long r = threadIdx.x; r < rows; r += blockDim.x) {
-        const float g = bf2f(dz[r*H + j]);
-        s0 += g*x[3*r]; s1 += g*x[3*r + 1]; s2 += g*x[3*r + 2];
+    long r = r0;
+    for (; r + 8 <= r1; r += 8) {
+        float gz[8];
+        #pragma unroll
+        for (int e = 0; e < 8; e++) gz[e] = bf2f(dz[(r + e)*H + j]);
+        #pragma unroll
+        for (int e = 0; e < 8; e++) {
+            s0 += gz[e]*x[3*(r + e)]; s1 += gz[e]*x[3*(r + e) + 1];
+            s2 += gz[e]*x[3*(r + e) + 2];
+        }
     }
-    __shared__ float red[3][TPB];
-    red[0][threadIdx.x] = s0; red[1][threadIdx.x] = s1; red[2][threadIdx.x] = s2;
+    for (; r < r1; r++) {
+        const float gz = bf2f(dz[r*H + j]);
+        s0 += gz*x[3*r]; s1 += gz*x[3*r + 1]; s2 += gz*x[3*r + 2];
+    }
+    red[0][u][j] = s0; red[1][u][j] = s1; red[2][u][j] = s2;
     __syncthreads();
-    for (int o = TPB/2; o > 0; o >>= 1) {
-        if ((int) threadIdx.x < o)
-            for (int q = 0; q < 3; q++)
-                red[q][threadIdx.x] += red[q][threadIdx.x + o];
-        __syncthreads();
+    if (u < 3) {
+        const float v = ((red[u][0][j] + red[u][1][j]) + red[u][2][j])
+                      + red[u][3][j];
+        out[(long) blockIdx.x*3*H + 3*j + u] = v;
     }
-    if (threadIdx.x < 3) dW1[3*j + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Adam on the FP32 master parameters; refreshes the BF16 copies.
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g,
-                       float* __restrict__ m, float* __restrict__ v, long n,
-                       float lr, float b1, float b2, float eps, float c1,
-                       float c2, bf16_t* __restrict__ pb)
+// Parameter gradients from their parts, then (adam != 0) the Adam step.
+// Segment q covers parameters [off, off + len): gradient i is the sum of
+// src[z*pstride + i - off] over its parts z in increasing order (split-K
+// partial products, per-row-block column sums, per-chunk first-layer sums),
+// so the result does not depend on timing.  The FP32 master parameters are
+// updated and their BF16 copies refreshed.
+struct GradTable {
+    FteikGradSeg s[kSurMaxSeg];
+    int nseg;
+};
+
+__global__ void k_grad_adam(GradTable tab, float* __restrict__ g,
+                            float* __restrict__ p, float* __restrict__ m,
+                            float* __restrict__ v, bf16_t* __restrict__ pb,
+                            long n, int adam, float lr, float b1, float b2,
+                            float eps, float c1, float c2)
 {
     GRID_STRIDE(i, n) {
-        const float gi = g[i];
-        const float mi = b1*m[i] + (1.0f - b1)*gi;
-        const float vi = b2*v[i] + (1.0f - b2)*gi*gi;
-        m[i] = mi; v[i] = vi;
-        const float pi = p[i] - lr*(mi*c1)/(sqrtf(vi*c2) + eps);
-        p[i] = pi;
-        pb[i] = f2bf(pi);
+        int q = 0;
+        while (q + 1 < tab.nseg && i >= tab.s[q + 1].off) q++;
+        const long o = i - tab.s[q].off;
+        const float* src = tab.s[q].src;
+        const long ps = tab.s[q].pstride;
+        const int np = tab.s[q].nparts;
+        float gi = 0.0f;
+        int z = 0;
+        for (; z + 8 <= np; z += 8) {
+            float u[8];
+            #pragma unroll
+            for (int e = 0; e < 8; e++) u[e] = src[(long) (z + e)*ps + o];
+            #pragma unroll
+            for (int e = 0; e < 8; e++) gi += u[e];
+        }
+        for (; z < np; z++) gi += src[(long) z*ps + o];
+        g[i] = gi;
+        if (adam) {
+            const float mi = b1*m[i] + (1.0f - b1)*gi;
+            const float vi = b2*v[i] + (1.0f - b2)*gi*gi;
+            m[i] = mi; v[i] = vi;
+            const float pi = p[i] - lr*(mi*c1)/(sqrtf(vi*c2) + eps);
+            p[i] = pi;
+            pb[i] = f2bf(pi);
+        }
     }
 }
 
@@ -357,24 +572,28 @@ __device__ __forceinline__ float t0_ray(const float* p, int s, const GeoArgs& g,
 
 // Training samples: rows r of step `step` take cell hash(seed, index) of a
 // pool of `pool` uniformly drawn cells; inputs and scaled residual targets.
+// One thread per (row, station): the table reads of a row are independent
+// gathers spread over the whole GPU.  Clears *loss for the step.
 __global__ void k_samples(const float* __restrict__ tab, long ldgrd, GeoArgs g,
                           long ngrd, unsigned long long seed, long pool,
                           long first, long rows, float* __restrict__ x,
-                          float* __restrict__ tgt, int ldt)
+                          float* __restrict__ tgt, int ldt,
+                          double* __restrict__ loss)
 {
-    GRID_STRIDE(r, rows) {
+    if (loss && blockIdx.x == 0 && threadIdx.x == 0) *loss = 0.0;
+    GRID_STRIDE(i, rows*ldt) {
+        const long r = i / ldt;
+        const int s = (int) (i - r*ldt);
         const long idx = (first + r) % pool;
         const long c = (long) (mix64(seed*0x100000001b3ull + (unsigned long long) idx)
                                % (unsigned long long) ngrd);
         float p[3];
         node_xyz(c, g, p);
-        x[3*r] = p[0]; x[3*r + 1] = p[1]; x[3*r + 2] = p[2];
-        for (int s = 0; s < ldt; s++) {
-            float v = 0.0f;
-            if (s < g.ntf)
-                v = (tab[(long) s*ldgrd + c] - t0_ray(p, s, g, nullptr))/g.scale;
-            tgt[r*ldt + s] = v;
-        }
+        if (s == 0) { x[3*r] = p[0]; x[3*r + 1] = p[1]; x[3*r + 2] = p[2]; }
+        float v = 0.0f;
+        if (s < g.ntf)
+            v = (tab[(long) s*ldgrd + c] - t0_ray(p, s, g, nullptr))/g.scale;
+        tgt[r*ldt + s] = v;
     }
 }
 
@@ -402,10 +621,10 @@ __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
 
 // Hidden layer of the inversion network, fused: 4 row blocks (primal and
 // three tangents) of R rows, H = 256.  z = h_in W^T on the MFMA units with
-// FP32 accumulation in the same k order as k_gemm_bf16 (so z is bitwise
+// FP32 accumulation in the same k order as k_gemm (so z is bitwise
 // the GEMM's), then h = silu(z + b) for the primal block and
 // h_d = silu'(z_0 + b) z_d for the tangents, rounded to BF16: what
-// k_gemm_bf16 followed by k_inv_act computes, without the FP32 round trip
+// k_gemm followed by k_inv_act computes, without the FP32 round trip
 // through memory.  One workgroup per 32 rows; wave d computes block d with
 // its A rows held in registers over the whole K, the weights stream from
 // L2 as 16-byte vectors, and the primal wave hands silu' to the tangent
@@ -664,37 +883,75 @@ int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
                       hipStream_t s)
 {
     if (M < 1 || N < 1 || K < 1) return 0;
-    const int tiles = ((N + GT - 1)/GT)*((M + GT - 1)/GT);
-    // few output tiles over a long k (the weight gradients: 256 x 256 over
-    // the batch rows): split k so the GPU's 256 CUs all work, then sum the
-    // partial products in a fixed order (the result does not depend on
-    // timing)
-    int nsplit = 1;
-    if (tiles < 256 && K >= 2048)
-        nsplit = std::min(std::max(512/tiles, 1), K/256);
-    const int kper = ((K + nsplit - 1)/nsplit + GK - 1)/GK*GK;
-    nsplit = (K + kper - 1)/kper;
-    float* P = nullptr;
-    if (nsplit > 1) {
-        const size_t need = (size_t) nsplit*M*N;
-        if (need > g_split_cap) {
-            if (g_split) hipFree(g_split);
-            g_split = nullptr;
-            g_split_cap = 0;
-            if (hipMalloc((void**) &g_split, need*sizeof(float)) != hipSuccess)
-                return 1;
-            g_split_cap = need;
-        }
-        P = g_split;
+    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
+    const int nsplit = gemm_split(M, N, K, &g.kper);
+    g.C = C; g.ldc = ldc; g.bias = bias; g.accumulate = accumulate;
+    if (nsplit == 1) return launch_gemm<EP_STORE>(g, 1, s);
+    const size_t need = (size_t) nsplit*M*N;
+    if (need > g_split_cap) {
+        if (g_split) hipFree(g_split);
+        g_split = nullptr;
+        g_split_cap = 0;
+        if (hipMalloc((void**) &g_split, need*sizeof(float)) != hipSuccess)
+            return 1;
+        g_split_cap = need;
     }
-    dim3 grid((N + GT - 1)/GT, (M + GT - 1)/GT, nsplit);
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s,
-                       (const bf16_t*) A, sam, sak, (const bf16_t*) B, sbk,
-                       sbn, C, ldc, M, N, K, bias, accumulate, kper, P);
-    if (nsplit > 1)
-        hipLaunchKernelGGL(k_gemm_reduce, dim3(nblocks((long) M*N)), dim3(TPB),
-                           0, s, P, nsplit, M, N, C, ldc, bias, accumulate);
+    g.P = g_split;
+    if (launch_gemm<EP_SPLIT>(g, nsplit, s)) return 1;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3(nblocks((long) M*N)), dim3(TPB),
+                       0, s, g_split, nsplit, M, N, C, ldc, bias, accumulate);
     return ok();
+}
+
+int fteik_k_gemm_parts(int M, int N, int K)
+{
+    int kper;
+    return gemm_split(M, N, K, &kper);
+}
+
+int fteik_k_gemm_partial(const void* A, long sam, long sak, const void* B,
+                         long sbk, long sbn, int M, int N, int K, float* P,
+                         hipStream_t s)
+{
+    if (M < 1 || N < 1 || K < 1) return 0;
+    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
+    const int nsplit = gemm_split(M, N, K, &g.kper);
+    g.P = P;
+    return launch_gemm<EP_SPLIT>(g, nsplit, s);
+}
+
+int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
+                      long sbk, long sbn, int M, int N, int K,
+                      const float* bias, float* z, void* h, long ld,
+                      hipStream_t s)
+{
+    if (M < 1 || N < 1 || K < 1 || K > GKW) return 2;
+    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
+    g.bias = bias; g.C = z; g.hout = (bf16_t*) h; g.ldc = ld;
+    return launch_gemm<EP_SILU>(g, 1, s);
+}
+
+int fteik_k_gemm_silu_bwd(const void* A, long sam, long sak, const void* B,
+                          long sbk, long sbn, int M, int N, int K,
+                          const float* z, void* dz, long ld, float* colp,
+                          hipStream_t s)
+{
+    if (M < 1 || N < 1 || K < 1 || K > GKW) return 2;
+    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
+    g.zin = z; g.hout = (bf16_t*) dz; g.ldc = ld; g.colp = colp;
+    return launch_gemm<EP_SILU_BWD>(g, 1, s);
+}
+
+int fteik_k_gemm_mse(const void* A, long sam, long sak, const void* B,
+                     long sbk, long sbn, int M, int N, int K,
+                     const float* bias, const float* tgt, void* dy, long ld,
+                     float inv, double* loss, float* colp, hipStream_t s)
+{
+    if (M < 1 || N < 1 || K < 1 || K > GKW) return 2;
+    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
+    g.bias = bias; g.tgt = tgt; g.hout = (bf16_t*) dy; g.ldc = ld;
+    g.inv = inv; g.loss = loss; g.colp = colp;
+    return launch_gemm<EP_MSE>(g, 1, s);
 }
 
 void fteik_k_sur_release(void)
@@ -719,62 +976,39 @@ int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
     return ok();
 }
 
-int fteik_k_sur_silu(const float* z, void* h, long n, hipStream_t s)
+int fteik_k_sur_dw1_parts(const void* dz, const float* x, long rows, int H,
+                          int nchunks, float* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_silu, dim3(nblocks(n)), dim3(TPB), 0, s, z,
-                       (bf16_t*) h, n);
+    if (H < 1 || H > 256 || nchunks < 1) return 2;
+    const long per = (rows + nchunks - 1)/nchunks;
+    hipLaunchKernelGGL(k_dw1_part, dim3(nchunks), dim3(H, 4), 0, s,
+                       (const bf16_t*) dz, x, rows, H, per, out);
     return ok();
 }
 
-int fteik_k_sur_silu_bwd(const float* dh, const float* z, void* dz, long n,
-                         hipStream_t s)
+int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
+                          float* p, float* m, float* v, void* pb, long n,
+                          int adam, float lr, float b1, float b2, float eps,
+                          float c1, float c2, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_silu_bwd, dim3(nblocks(n)), dim3(TPB), 0, s, dh, z,
-                       (bf16_t*) dz, n);
-    return ok();
-}
-
-int fteik_k_sur_mse(const float* y, const float* tgt, void* dy, long n,
-                    float inv, double* loss, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mse, dim3(nblocks(n) < 1024 ? nblocks(n) : 1024),
-                       dim3(TPB), 0, s, y, tgt, (bf16_t*) dy, n, inv, loss);
-    return ok();
-}
-
-int fteik_k_sur_colsum(const void* d, long rows, int cols, float* out,
-                       hipStream_t s)
-{
-    hipLaunchKernelGGL(k_colsum, dim3(cols), dim3(TPB), 0, s,
-                       (const bf16_t*) d, rows, cols, out);
-    return ok();
-}
-
-int fteik_k_sur_dw1(const void* dz, const float* x, long rows, int H,
-                    float* dW1, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_dw1, dim3(H), dim3(TPB), 0, s, (const bf16_t*) dz, x,
-                       rows, H, dW1);
-    return ok();
-}
-
-int fteik_k_sur_adam(float* p, const float* g, float* m, float* v, long n,
-                     float lr, float b1, float b2, float eps, float c1,
-                     float c2, void* pb, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_adam, dim3(nblocks(n)), dim3(TPB), 0, s, p, g, m, v,
-                       n, lr, b1, b2, eps, c1, c2, (bf16_t*) pb);
+    if (nseg < 1 || nseg > kSurMaxSeg) return 2;
+    GradTable tab = {};
+    for (int q = 0; q < nseg; q++) tab.s[q] = segs[q];
+    tab.nseg = nseg;
+    hipLaunchKernelGGL(k_grad_adam, dim3(nblocks(n)), dim3(TPB), 0, s, tab,
+                       g, p, m, v, (bf16_t*) pb, n, adam, lr, b1, b2, eps,
+                       c1, c2);
     return ok();
 }
 
 int fteik_k_sur_samples(const float* tab, long ldgrd, const void* geo,
                         long ngrd, unsigned long long seed, long pool,
                         long first, long rows, float* x, float* tgt, int ldt,
-                        hipStream_t s)
+                        double* loss, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_samples, dim3(nblocks(rows)), dim3(TPB), 0, s, tab,
-                       ldgrd, *(const GeoArgs*) geo, ngrd, seed, pool, first,
-                       rows, x, tgt, ldt);
+    hipLaunchKernelGGL(k_samples, dim3(nblocks(rows*ldt)), dim3(TPB), 0, s,
+                       tab, ldgrd, *(const GeoArgs*) geo, ngrd, seed, pool,
+                       first, rows, x, tgt, ldt, loss);
     return ok();
 }
 

@@ -86,8 +86,11 @@ def test_parameter_count():
     L.fteik_surrogate_free()
 
 
-def test_forward_and_gradients_match_cpu_statement():
-    ntf, n = 5, 37
+@pytest.mark.parametrize("ntf,n", [(5, 37), (48, 4100)])
+def test_forward_and_gradients_match_cpu_statement(ntf, n):
+    """n = 4100 takes the training kernels' multi-part paths: the weight
+    gradients split over 15 row ranges, the bias gradients over 65 row
+    blocks (the last one of 4 rows), all summed by the Adam kernel."""
     L, npar = _init_surrogate(ntf)
     ierr = c_int(1)
     rng = np.random.default_rng(11)
