@@ -10,6 +10,7 @@
 // cross-check and serves grids wider than 513 nodes.  Several sources
 // (solveLSM) run as independent workgroups of one launch.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "stencils.h"
 #include "kernels.h"
 
@@ -124,11 +125,17 @@ void solve2d_kernel(double* __restrict__ ttall, const double* __restrict__ slow,
 // step.  Lane 0 of wave w > 0 takes lane 63 of wave w - 1 from a ring in LDS
 // after that wave's step counter has passed the step; wave 0's lane 0 reads
 // the frozen column left of the box.  Own old times and slownesses are
-// loaded H2D steps ahead.  The order of the node updates is a topological
-// order of each sweep's dependency graph, as in the level kernel, so the
-// results are the same bit for bit.
+// staged through LDS a batch of S2 steps at a time, loaded cooperatively
+// along the columns (a load instruction touches a few cache lines instead
+// of one per lane), and the new times leave the same way.  The order of
+// the node updates is a topological order of each sweep's dependency
+// graph, as in the level kernel, so the results are the same bit for bit.
 constexpr int R2D = 16;      // ring slots: wave w may lead wave w + 1 by that
 constexpr int T2P = 512;     // largest workgroup (8 waves)
+constexpr int S2 = 8;        // steps per staged batch
+constexpr int CW2 = S2 + 1;  // cell rows of a staged cell column
+constexpr int NRT = (65*S2 + 63)/64;    // old-time pieces per lane
+constexpr int NRC = (65*CW2 + 63)/64;   // cell pieces per lane
 
 __device__ __forceinline__ double wave_shr1(double x)
 {
@@ -140,8 +147,8 @@ __device__ __forceinline__ double wave_shr1(double x)
     return __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
 }
 
-// TMAX threads at most (TMAX/64 waves); H2D steps per prefetch batch
-template <int TMAX, int H2D>
+// TMAX threads at most (TMAX/64 waves)
+template <int TMAX>
 __global__ __launch_bounds__(TMAX)
 void solve2d_pipe_kernel(double* __restrict__ ttall,
                          const double* __restrict__ slow, int nz, int nx,
@@ -158,6 +165,12 @@ void solve2d_pipe_kernel(double* __restrict__ ttall,
     __shared__ Consts<double> cs;
     __shared__ double ring[TMAX/64][R2D];
     __shared__ int s_done[TMAX/64];
+    // per wave: old times of a batch (64 node columns + the frozen left
+    // column), its cell window (65 columns x CW2 rows), its new times;
+    // rows padded against LDS bank conflicts
+    __shared__ double s_t[TMAX/64][65][S2 + 1];
+    __shared__ double s_c[TMAX/64][65][CW2 + 2];
+    __shared__ double s_o[TMAX/64][64][S2 + 1];
     __shared__ int s_fail;
     if (threadIdx.x == 0) {
         // localSolver.F90:299-345
@@ -214,113 +227,167 @@ void solve2d_pipe_kernel(double* __restrict__ ttall,
         if (lane == 0 && wv < nwv) s_done[wv] = Lf - 1;
         __syncthreads();
         if (s_fail) break;                                // uniform
+        auto sweep_columns = [&](auto kind) {
+        constexpr bool LINIT = decltype(kind)::value;
         if (wv < wbox) {
-            const int fj = fx0 + 64*wv + lane;
+            // This wave's columns are F .. F + 63 (flipped), lane j has
+            // column fj = F + j and at step L the node on flipped row
+            // fi = L - fj.  In flipped cells (cell column fj - 1 lies
+            // upwind of node column fj, cell row fi - 1 upwind of node row
+            // fi) a node reads the cells (fj-1, fi-1), (min(fj, nx-2),
+            // fi-1) and (fj-1, min(fi, nz-2)) (solver2D.F90:491-502: slow
+            // at izc over xa, xb and at ixc over max(iz-1,1), min(iz,nz-1)).
+            // Per batch of S2 steps the wave loads, S2 lanes along each
+            // column: the S2 old times of each of its 64 node columns (and,
+            // wave 0, of the frozen column left of the box) and S2 + 1 rows
+            // of each of its 65 cell columns F-1 .. F+63 (cell column c
+            // from row L0 - (F-1+c) - 2), into LDS one batch ahead through
+            // registers; the new times of a batch go to LDS and out to the
+            // field S2 lanes per column after it.  No lane reads the field
+            // values another lane of this sweep writes (old times are the
+            // lane's own, ahead of its writes), so deferring the stores to
+            // the end of a batch changes nothing the sweep reads.
+            const int F = fx0 + 64*wv;
+            const int fj = F + lane;
             const bool colA = fj <= fx1;
             const int cfj = min(fj, fx1);
             const int ix = (sgntx > 0) ? cfj + 1 : nx - cfj;
-            double* const col = tt + (long) (ix - 1)*nz;
-            // the frozen column left of the box (wave 0, lane 0)
-            const double* const lcol = tt + (long) (ix - sgntx - 1)*nz;
-            const int xa = max(ix - 1, 1), xb = min(ix, nx - 1);
-            const int ixc = ix - (sgntx > 0 ? 1 : 0);
-            const double* const sa = slow + (long) (xa - 1)*(nz - 1);
-            const double* const sb = slow + (long) (xb - 1)*(nz - 1);
-            const double* const sc = slow + (long) (ixc - 1)*(nz - 1);
+            auto nodecol = [&](int f) -> const double* {   // column f, clamped
+                f = min(f, fx1);
+                return tt + (long) (((sgntx > 0) ? f + 1 : nx - f) - 1)*nz;
+            };
+            const double* const lcol = tt + (long) (((sgntx > 0) ? fx0 + 1
+                                                     : nx - fx0) - sgntx - 1)*nz;
             auto zoff = [&](int fi) -> int {        // 0-based node in column
                 fi = min(max(fi, fz0 - 1), fz1);
                 return (sgntz > 0) ? fi : nz - 1 - fi;
             };
-            // step s's own old time, the left frozen value and the
-            // slownesses (solver2D.F90:491-502): a[0..5] = t, tleft,
-            // slow(xa, izc), slow(xb, izc), slow(ixc, max(iz-1,1)),
-            // slow(ixc, min(iz,nz-1))
-            double nb[H2D][6];
-            auto load = [&](int s0) {
+            auto cell = [&](int fc, int fr) -> const double* {  // flipped cell
+                fc = min(fc, nx - 2);
+                fr = min(max(fr, 0), nz - 2);
+                const int c0 = (sgntx > 0) ? fc : nx - 2 - fc;
+                const int r0 = (sgntz > 0) ? fr : nz - 2 - fr;
+                return slow + (long) c0*(nz - 1) + r0;
+            };
+            double (*const st)[S2 + 1] = s_t[wv];
+            double (*const sc)[CW2 + 2] = s_c[wv];
+            double (*const so)[S2 + 1] = s_o[wv];
+            double rt[NRT], rc[NRC];
+            // batch from step L0 into registers: old-time piece i is
+            // column lane/S2 + (64/S2) i, element lane%S2 (column 64:
+            // wave 0's frozen left column; pieces past it are harmless
+            // in-range reads); cell piece i is slot lane + 64 i of the
+            // 65 x CW2 cell window
+            auto fetch = [&](int L0) {
                 #pragma unroll
-                for (int k = 0; k < H2D; k++) {
-                    const int fi = s0 + k - fj;
-                    const int zn = zoff(fi);
-                    const int iz = zn + 1;
-                    const int izc = min(max(iz - (sgntz > 0 ? 1 : 0), 1),
-                                        nz - 1);
-                    nb[k][0] = col[zn];
-                    nb[k][1] = lcol[zn];       // used by wave 0, lane 0
-                    nb[k][2] = sa[izc - 1];
-                    nb[k][3] = sb[izc - 1];
-                    nb[k][4] = sc[max(iz - 1, 1) - 1];
-                    nb[k][5] = sc[min(iz, nz - 1) - 1];
+                for (int i = 0; i < NRT; i++) {
+                    const int c = lane/S2 + (64/S2)*i, e = lane % S2;
+                    const int cc = min(c, 63);
+                    const double* p = nodecol(F + cc) + zoff(L0 - F - cc + e);
+                    if (i == NRT - 1 && wv == 0 && c == 64)
+                        p = lcol + zoff(L0 - fx0 + e);
+                    rt[i] = *p;
+                }
+                #pragma unroll
+                for (int i = 0; i < NRC; i++) {
+                    const int q = min(lane + 64*i, 65*CW2 - 1);
+                    const int c = q / CW2, e = q - c*CW2;
+                    rc[i] = *cell(F - 1 + c, L0 - (F - 1 + c) - 2 + e);
                 }
             };
-            double P = col[zoff(fz0 - 1)];          // node above the box
+            auto park = [&]() {                     // registers to LDS
+                #pragma unroll
+                for (int i = 0; i < NRT; i++) {
+                    const int c = lane/S2 + (64/S2)*i, e = lane % S2;
+                    if (c <= 64) st[c][e] = rt[i];
+                }
+                #pragma unroll
+                for (int i = 0; i < NRC; i++) {
+                    const int q = lane + 64*i;
+                    const int c = q / CW2, e = q - c*CW2;
+                    if (q < 65*CW2) sc[c][e] = rc[i];
+                }
+            };
+            // the new times of the batch from step L0 to the field; lanes
+            // with no node to write store to their scratch slot, so every
+            // lane issues the same stores
+            double* const scr = scratch + (long) src*T2P + threadIdx.x;
+            auto flush = [&](int L0) {
+                #pragma unroll
+                for (int i = 0; i < S2; i++) {
+                    const int c = lane/S2 + (64/S2)*i, e = lane % S2;
+                    const int fi = L0 - F - c + e;
+                    const bool act = F + c <= fx1 && fi >= fz0 && fi <= fz1;
+                    double* const dst = act ? (double*) nodecol(F + c)
+                        + ((sgntz > 0) ? fi : nz - 1 - fi) : scr;
+                    *dst = so[c][e];
+                }
+            };
+            double P = *(nodecol(fj) + zoff(fz0 - 1));  // node above the box
             double tep = 0.0;
             int cl = Lf - 2, cr = -(1 << 30);       // cached counters
-            load(Lf);
-            for (int L0 = Lf; L0 <= Ll; L0 += H2D) {
-                double cur[H2D][6];
+            fetch(Lf);
+            for (int L0 = Lf; L0 <= Ll; L0 += S2) {
+                park();
+                if (L0 > Lf) flush(L0 - S2);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_wave_barrier();
+                if (L0 + S2 <= Ll) fetch(L0 + S2);
                 #pragma unroll
-                for (int k = 0; k < H2D; k++)
-                    #pragma unroll
-                    for (int m = 0; m < 6; m++) cur[k][m] = nb[k][m];
-                if (L0 + H2D <= Ll) load(L0 + H2D);
-                #pragma unroll
-                for (int k = 0; k < H2D; k++) {
+                for (int k = 0; k < S2; k++) {
                     const int L = L0 + k;
                     if (L > Ll) break;                   // wave-uniform
                     const int fi = L - fj;
                     double te = wave_shr1(P);
                     if (wv == 0) {
-                        if (lane == 0) te = cur[k][1];
+                        te = (lane == 0) ? st[64][k] : te;
                     } else if (L - 1 <= fz1 + fx0 + 64*wv - 1) {
                         // lane 63 of the wave before, after its step L - 1
-                        unsigned long long spins = 0;
+                        // (counters are read as wave-uniform values)
+                        unsigned spins = 0;
                         while (cl < L - 1) {
-                            cl = __hip_atomic_load(&s_done[wv - 1],
-                                                   __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            cl = __builtin_amdgcn_readfirstlane(
+                                __hip_atomic_load(&s_done[wv - 1],
+                                                  __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP));
                             if (cl >= L - 1) break;
                             if (++spins > kSpinLimit2D) { fail = true; break; }
                             __builtin_amdgcn_s_sleep(1);
                         }
                         __atomic_signal_fence(__ATOMIC_SEQ_CST);
                         const double rv = ring[wv - 1][L % R2D];
-                        if (lane == 0) te = rv;
+                        te = (lane == 0) ? rv : te;
                     }
                     const double tev = tep;
                     if (colA && fi >= fz0 && fi <= fz1) {
                         const int iz = (sgntz > 0) ? fi + 1 : nz - fi;
-                        const double s1 = rmin(cur[k][2], cur[k][3]);
-                        const double s2 = rmin(cur[k][4], cur[k][5]);
-                        const double s3 = cur[k][2 + (ixc == xa ? 0 : 1)];
-                        const double t = cur[k][0];
+                        const double cA = sc[lane][k];          // (fj-1, fi-1)
+                        const double cB = sc[lane][k + 1];      // (fj-1, min(fi, nz-2))
+                        const double cC = (fj <= nx - 2) ? sc[lane + 1][k + 1]
+                                                         : cA;  // (min(fj, nx-2), fi-1)
+                        const double s1 = rmin(cA, cC);
+                        const double s2 = rmin(cA, cB);
+                        const double s3 = cA;
+                        const double t = st[lane][k];
                         double tupd;
-                        if (linit)
+                        if constexpr (LINIT)
                             tupd = init2d(c, sgntz, sgntx, iz, ix, P, te, tev,
                                           t, s1, s2, s3);
                         else
                             tupd = noinit2d(c, P, te, tev, s1, s2, s3);
                         P = rmin(t, tupd);
                     }
-                    // one store per step in every lane (lanes off the box
-                    // write their slot of the scratch line), so the compiler
-                    // counts this wave's memory operations exactly and the
-                    // prefetched loads are waited for without these stores
-                    {
-                        const bool act = colA && fi >= fz0 && fi <= fz1;
-                        const int iz = (sgntz > 0) ? fi + 1 : nz - fi;
-                        double* const dst = act ? col + (iz - 1)
-                            : scratch + (long) src*T2P + threadIdx.x;
-                        *dst = P;
-                    }
+                    so[lane][k] = P;
                     tep = te;
                     // lane 63's value for the next wave's step L + 1, once
                     // that wave is past step L + 1 - R2D (the slot's reader)
                     if (wv + 1 < wbox) {
-                        unsigned long long spins = 0;
+                        unsigned spins = 0;
                         while (cr < L + 1 - R2D) {
-                            cr = __hip_atomic_load(&s_done[wv + 1],
-                                                   __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            cr = __builtin_amdgcn_readfirstlane(
+                                __hip_atomic_load(&s_done[wv + 1],
+                                                  __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP));
                             if (cr >= L + 1 - R2D) break;
                             if (++spins > kSpinLimit2D) { fail = true; break; }
                             __builtin_amdgcn_s_sleep(1);
@@ -338,7 +405,17 @@ void solve2d_pipe_kernel(double* __restrict__ ttall,
                 }
                 if (fail) break;
             }
+            // the last batch's new times
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_wave_barrier();
+            if (!fail) {
+                const int nb = (Ll - Lf)/S2;
+                flush(Lf + nb*S2);
+            }
         }
+        };
+        if (linit) sweep_columns(std::true_type{});
+        else sweep_columns(std::false_type{});
         if (fail && lane == 0) {
             atomicExch(err, 1);
             s_fail = 1;
@@ -363,7 +440,7 @@ extern "C" int fteik_k_solve2d(double* tt, const double* slow, int nz, int nx,
     // pipelined columns when every flipped column of a sweep box has a lane
     if (schedule == 0 && nx - 1 <= T2P && err && scratch) {
         const int threads = 64*((nx - 1 + 63)/64);
-        hipLaunchKernelGGL((solve2d_pipe_kernel<T2P, 4>), dim3(nsrc_batch),
+        hipLaunchKernelGGL((solve2d_pipe_kernel<T2P>), dim3(nsrc_batch),
                            dim3(threads), 0, stream, tt, slow, nz, nx, dz, dx,
                            eps, nsweep, (long) ld_tt, zsa_v, xsa_v, zsi_v,
                            xsi_v, err, scratch);

@@ -1,9 +1,13 @@
 // sweep3d.hip -- 3D level-set fast sweeping on gfx950: the single-grid
 // solver's kernels and launchers.  One sweep direction is one launch of the
 // pipelined column-tile kernel (sweep_pencil.h); kernel boundaries between
-// sweeps give the coherence the next sweep needs.  Launches are cooperative
-// (every tile resident) so waits between tiles always end; schedule 1 is a
-// one-launch-per-level cross-check and fallback.
+// sweeps give the coherence the next sweep needs.  A launch is cooperative
+// when all its tiles fit on the GPU at once; otherwise (the default 8 x 8
+// tiles at 257^3) it relies on in-order workgroup dispatch: a tile waits
+// only on lower-numbered tiles, so the lowest unfinished one is always
+// resident and every wait ends.  Each wait also has a spin limit that turns
+// a hang into a reported error.  Schedule 1 is a one-launch-per-level
+// cross-check and fallback.
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <stdint.h>
