@@ -19,15 +19,21 @@ struct Surrogate {
     long oW1, ob1, oW[3], ob[3], oW5, ob5;
     float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
     unsigned short *d_pb = nullptr;    // BF16 copy of all parameters
+    // BF16 transposes of the weights ([in][out]; W5's rows padded to TP
+    // outputs with zeros) for the backward products
+    unsigned short *d_pbT = nullptr;
+    long oWT[3] = {0, 0, 0}, oW5T = 0;
+    int TP = 16;
     // work buffers (grown on demand)
     long cap = 0;
     float *d_x = nullptr, *d_tgt = nullptr, *d_y = nullptr;
     float *d_z[4] = {nullptr, nullptr, nullptr, nullptr};
     unsigned short *d_h[4] = {nullptr, nullptr, nullptr, nullptr};
-    unsigned short *d_dy = nullptr, *d_dza = nullptr, *d_dzb = nullptr;
+    unsigned short *d_dy = nullptr;
+    unsigned short *d_dz[3] = {nullptr, nullptr, nullptr};   // dz_1..dz_3
     // gradient parts: split-K partial products of the weight gradients
-    // (grown on demand), column sums per 64-row block of the bias
-    // gradients (5 slots), first-layer sums per row chunk
+    // (grown on demand), column sums per 32-row block of the bias
+    // gradients (5 slots), first-layer sums per 32-row block
     float *d_P = nullptr, *d_colp = nullptr, *d_dw1p = nullptr;
     long capP = 0;
     double *d_loss = nullptr;
@@ -88,7 +94,8 @@ void free_surrogate()
 {
     dfree(gs.d_geo); dfree(gs.d_p); dfree(gs.d_g); dfree(gs.d_m); dfree(gs.d_v);
     dfree(gs.d_pb); dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y);
-    dfree(gs.d_dy); dfree(gs.d_dza); dfree(gs.d_dzb); dfree(gs.d_P);
+    dfree(gs.d_dy); dfree(gs.d_P); dfree(gs.d_pbT);
+    for (int l = 0; l < 3; l++) dfree(gs.d_dz[l]);
     dfree(gs.d_colp); dfree(gs.d_dw1p); dfree(gs.d_loss);
     dfree(gs.d_zero);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
@@ -97,22 +104,21 @@ void free_surrogate()
     gs = Surrogate();
 }
 
-constexpr int kDw1Chunks = 64;        // row chunks of the first layer's sums
-
-long row_blocks(long rows) { return (rows + 63)/64; }
+long row_blocks(long rows) { return (rows + kMlpRows - 1)/kMlpRows; }
 
 bool sur_work(long rows)
 {
     if (rows <= gs.cap) return true;
     dfree(gs.d_x); dfree(gs.d_tgt); dfree(gs.d_y); dfree(gs.d_dy);
-    dfree(gs.d_dza); dfree(gs.d_dzb); dfree(gs.d_colp); dfree(gs.d_dw1p);
+    dfree(gs.d_colp); dfree(gs.d_dw1p);
+    for (int l = 0; l < 3; l++) dfree(gs.d_dz[l]);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     const size_t R = (size_t) rows, H = (size_t) gs.H, T = (size_t) gs.ntf;
+    const size_t nb = (size_t) row_blocks(rows);
     bool ok = dalloc(&gs.d_x, 3*R) && dalloc(&gs.d_tgt, R*T) &&
               dalloc(&gs.d_y, R*T) && dalloc(&gs.d_dy, R*T) &&
-              dalloc(&gs.d_dza, R*H) && dalloc(&gs.d_dzb, R*H) &&
-              dalloc(&gs.d_colp, 5*(size_t) row_blocks(rows)*H) &&
-              dalloc(&gs.d_dw1p, (size_t) kDw1Chunks*3*H);
+              dalloc(&gs.d_colp, 5*nb*H) && dalloc(&gs.d_dw1p, nb*3*H);
+    for (int l = 0; l < 3 && ok; l++) ok = dalloc(&gs.d_dz[l], R*H);
     for (int l = 0; l < 4 && ok; l++)
         ok = dalloc(&gs.d_z[l], R*H) && dalloc(&gs.d_h[l], R*H);
     if (!ok) { gs.cap = 0; return false; }
@@ -120,11 +126,23 @@ bool sur_work(long rows)
     return true;
 }
 
-// Forward pass over `rows` rows of d_x.  Inference (train false): network
-// output (before the baseline and scale) into d_y.  Training: the output
-// layer's GEMM forms dy = d(mean squared error)/dy against d_tgt in its
-// epilogue (with the loss and the output bias gradient's parts) instead.
-bool sur_forward(long rows, bool train)
+// BF16 transposes of the weights from the BF16 copy (after it changed
+// other than by the Adam kernel, which keeps them current itself).
+bool sur_transposes()
+{
+    hipStream_t s = g_dev.stream;
+    const int H = gs.H;
+    bool ok = true;
+    for (int l = 0; l < 3 && ok; l++)
+        ok = fteik_k_sur_transpose(gs.d_pb + gs.oW[l], H, H,
+                                   gs.d_pbT + gs.oWT[l], H, s) == 0;
+    return ok && fteik_k_sur_transpose(gs.d_pb + gs.oW5, gs.ntf, H,
+                                       gs.d_pbT + gs.oW5T, gs.TP, s) == 0;
+}
+
+// Forward pass over `rows` rows of d_x: network output (before the
+// baseline and scale) into d_y.
+bool sur_forward(long rows)
 {
     hipStream_t s = g_dev.stream;
     const int H = gs.H, T = gs.ntf;
@@ -137,27 +155,20 @@ bool sur_forward(long rows, bool train)
                                (int) rows, H, H, gs.d_p + gs.ob[l],
                                gs.d_z[l + 1], gs.d_h[l + 1], H, s) == 0;
     }
-    if (!ok) return false;
-    if (!train)
-        return fteik_k_gemm_bf16(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, gs.d_y,
-                                 T, (int) rows, T, H, gs.d_p + gs.ob5, 0,
-                                 s) == 0;
-    const float inv = 1.0f/(float) ((double) rows*T);
-    return fteik_k_gemm_mse(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, (int) rows, T,
-                            H, gs.d_p + gs.ob5, gs.d_tgt, gs.d_dy, T, inv,
-                            gs.d_loss, gs.d_colp + 3*row_blocks(rows)*H,
-                            s) == 0;
+    return ok && fteik_k_gemm_bf16(gs.d_h[3], H, 1, pb + gs.oW5, 1, H, gs.d_y,
+                                   T, (int) rows, T, H, gs.d_p + gs.ob5, 0,
+                                   s) == 0;
 }
 
-// Backward pass after sur_forward(rows, true): the parts of every
-// parameter gradient (split-K weight-gradient products, bias column sums
-// per row block, first-layer sums per row chunk), and their segment
-// table for k_grad_adam in parameter order.
-bool sur_backward(long rows, FteikGradSeg *seg, int *nseg)
+// One training pass over `rows` rows of d_x against d_tgt (the mean
+// squared error over rows x ntf added to d_loss): the fused pass of
+// 32-row blocks through the network and back, then the weight-gradient
+// products as split-K parts; the segment table for k_grad_adam lists
+// every parameter gradient's parts in parameter order.
+bool sur_train_pass(long rows, FteikGradSeg *seg, int *nseg)
 {
     hipStream_t s = g_dev.stream;
     const int H = gs.H, T = gs.ntf, M = (int) rows;
-    const unsigned short *pb = gs.d_pb;
     const long nrb = row_blocks(rows);
     const int nsH = fteik_k_gemm_parts(H, H, M);
     const int ns5 = fteik_k_gemm_parts(T, H, M);
@@ -169,39 +180,46 @@ bool sur_backward(long rows, FteikGradSeg *seg, int *nseg)
         gs.capP = needP;
     }
     float *P5 = gs.d_P + 3L*nsH*H*H;
-    auto colp = [&](int slot) { return gs.d_colp + slot*nrb*H; };
-    // output layer: dW5 = dy^T h3; dz3 = (dy W5) silu'(z3), whose column
-    // sums are the bias gradient of the layer that made z3
-    bool ok = fteik_k_gemm_partial(gs.d_dy, 1, T, gs.d_h[3], H, 1, T, H, M,
-                                   P5, s) == 0 &&
-              fteik_k_gemm_silu_bwd(gs.d_dy, T, 1, pb + gs.oW5, H, 1, M, H, T,
-                                    gs.d_z[3], gs.d_dza, H, colp(2), s) == 0;
-    unsigned short *cur = gs.d_dza, *nxt = gs.d_dzb;
-    for (int l = 2; l >= 0 && ok; l--) {
-        // layer l+1 (weights oW[l]) maps h_l -> z_{l+1}; cur holds dz_{l+1}
-        ok = fteik_k_gemm_partial(cur, 1, H, gs.d_h[l], H, 1, H, H, M,
-                                  gs.d_P + (long) l*nsH*H*H, s) == 0 &&
-             fteik_k_gemm_silu_bwd(cur, H, 1, pb + gs.oW[l], H, 1, M, H, H,
-                                   gs.d_z[l], nxt, H,
-                                   colp(l >= 1 ? l - 1 : 4), s) == 0;
-        std::swap(cur, nxt);
+    FteikMlpArgs a;
+    memset(&a, 0, sizeof(a));
+    a.x = gs.d_x; a.tgt = gs.d_tgt; a.rows = rows; a.T = T; a.TP = gs.TP;
+    a.p = gs.d_p; a.pb = gs.d_pb; a.pbT = gs.d_pbT;
+    a.oW1 = gs.oW1; a.ob1 = gs.ob1; a.oW5 = gs.oW5; a.ob5 = gs.ob5;
+    a.oW5T = gs.oW5T;
+    for (int l = 0; l < 3; l++) {
+        a.oW[l] = gs.oW[l]; a.ob[l] = gs.ob[l]; a.oWT[l] = gs.oWT[l];
+        a.dz[l + 1] = gs.d_dz[l];
     }
-    ok = ok && fteik_k_sur_dw1_parts(cur, gs.d_x, rows, H, kDw1Chunks,
-                                     gs.d_dw1p, s) == 0;
+    for (int l = 0; l < 4; l++) { a.z[l] = gs.d_z[l]; a.h[l] = gs.d_h[l]; }
+    a.dy = gs.d_dy; a.colp = gs.d_colp; a.dw1p = gs.d_dw1p;
+    a.inv = 1.0f/(float) ((double) rows*T);
+    a.loss = gs.d_loss;
+    // weight gradients: dW5 = dy^T h3, dW_l = dz_{l+1}^T h_l
+    bool ok = fteik_k_mlp_train(&a, s) == 0 &&
+              fteik_k_gemm_partial(gs.d_dy, 1, T, gs.d_h[3], H, 1, T, H, M,
+                                   P5, s) == 0;
+    for (int l = 0; l < 3 && ok; l++)
+        ok = fteik_k_gemm_partial(gs.d_dz[l], 1, H, gs.d_h[l], H, 1, H, H, M,
+                                  gs.d_P + (long) l*nsH*H*H, s) == 0;
+    auto colp = [&](int slot) { return gs.d_colp + slot*nrb*H; };
     int q = 0;
-    auto add = [&](long off, long len, const float *src, int np, long ps) {
+    auto add = [&](long off, long len, const float *src, int np, long ps,
+                   unsigned short *tc, int ncol, int ldt) {
         seg[q].off = off; seg[q].len = len; seg[q].src = src;
         seg[q].nparts = np; seg[q].pstride = ps;
+        seg[q].tcopy = tc; seg[q].ncol = ncol; seg[q].ldt = ldt;
         q++;
     };
-    add(gs.oW1, 3L*H, gs.d_dw1p, kDw1Chunks, 3L*H);
-    add(gs.ob1, H, colp(4), (int) nrb, H);
+    add(gs.oW1, 3L*H, gs.d_dw1p, (int) nrb, 3L*H, nullptr, 0, 0);
+    add(gs.ob1, H, colp(4), (int) nrb, H, nullptr, 0, 0);
     for (int l = 0; l < 3; l++) {
-        add(gs.oW[l], (long) H*H, gs.d_P + (long) l*nsH*H*H, nsH, (long) H*H);
-        add(gs.ob[l], H, colp(l), (int) nrb, H);
+        add(gs.oW[l], (long) H*H, gs.d_P + (long) l*nsH*H*H, nsH, (long) H*H,
+            gs.d_pbT + gs.oWT[l], H, H);
+        add(gs.ob[l], H, colp(l), (int) nrb, H, nullptr, 0, 0);
     }
-    add(gs.oW5, (long) T*H, P5, ns5, (long) T*H);
-    add(gs.ob5, T, colp(3), (int) nrb, T);
+    add(gs.oW5, (long) T*H, P5, ns5, (long) T*H, gs.d_pbT + gs.oW5T, H,
+        gs.TP);
+    add(gs.ob5, T, colp(3), (int) nrb, T, nullptr, 0, 0);
     *nseg = q;
     return ok;
 }
@@ -266,10 +284,15 @@ void fteik_surrogate_initialize(int ntf, int nz, int nx, int ny, double dz,
     for (int l = 0; l < 3; l++) fill(gs.oW[l], H*H, (int) H);
     fill(gs.oW5, ntf*H, (int) H);
     for (long i = gs.oW5; i < gs.oW5 + ntf*H; i++) p[(size_t) i] *= 0.1f;
+    gs.TP = (ntf + 15)/16*16;
+    for (int l = 0; l < 3; l++) gs.oWT[l] = (long) l*H*H;
+    gs.oW5T = 3*H*H;
+    const size_t nT = (size_t) (3*H*H + H*gs.TP);
     bool ok = dalloc((char **) &gs.d_geo, sizeof(FteikSurrogateGeo)) &&
               dalloc(&gs.d_p, (size_t) o) && dalloc(&gs.d_g, (size_t) o) &&
               dalloc(&gs.d_m, (size_t) o) && dalloc(&gs.d_v, (size_t) o) &&
-              dalloc(&gs.d_pb, (size_t) o) && dalloc(&gs.d_loss, 1) &&
+              dalloc(&gs.d_pb, (size_t) o) && dalloc(&gs.d_pbT, nT) &&
+              dalloc(&gs.d_loss, 1) &&
               dalloc(&gs.d_zero, 64) &&
               h2d(gs.d_geo, &g, sizeof(g)) &&
               h2d(gs.d_p, p.data(), (size_t) o*sizeof(float));
@@ -277,7 +300,9 @@ void fteik_surrogate_initialize(int ntf, int nz, int nx, int ny, double dz,
         hipMemsetAsync(gs.d_m, 0, (size_t) o*sizeof(float), g_dev.stream);
         hipMemsetAsync(gs.d_v, 0, (size_t) o*sizeof(float), g_dev.stream);
         hipMemsetAsync(gs.d_zero, 0, 64*sizeof(float), g_dev.stream);
-        ok = fteik_k_f2bf(gs.d_p, gs.d_pb, o, g_dev.stream) == 0 && sync_ok();
+        hipMemsetAsync(gs.d_pbT, 0, nT*sizeof(unsigned short), g_dev.stream);
+        ok = fteik_k_f2bf(gs.d_p, gs.d_pb, o, g_dev.stream) == 0 &&
+             sur_transposes() && sync_ok();
     }
     if (!ok) { msg("%s: GPU error", who); free_surrogate(); *ierr = 1; return; }
     gs.init = true;
@@ -300,7 +325,8 @@ void fteik_surrogate_setParameters(int np, const float *p, int *ierr)
     *ierr = (gs.init && np == gs.np) ? 0 : 1;
     if (*ierr) return;
     if (!h2d(gs.d_p, p, (size_t) np*sizeof(float)) ||
-        fteik_k_f2bf(gs.d_p, gs.d_pb, np, g_dev.stream) != 0 || !sync_ok())
+        fteik_k_f2bf(gs.d_p, gs.d_pb, np, g_dev.stream) != 0 ||
+        !sur_transposes() || !sync_ok())
         *ierr = 1;
 }
 
@@ -309,8 +335,7 @@ void fteik_surrogate_forward(int n, const float *pos, float *out, int *ierr)
 {
     *ierr = 0;
     if (!gs.init || n < 1 || !sur_work(n) ||
-        !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) ||
-        !sur_forward(n, false) ||
+        !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) || !sur_forward(n) ||
         !d2h(out, gs.d_y, (size_t) n*gs.ntf*sizeof(float))) {
         msg("fteik_surrogate_forward: failed");
         *ierr = 1;
@@ -329,8 +354,8 @@ void fteik_surrogate_gradients(int n, const float *pos, const float *tgt,
     if (!gs.init || n < 1 || !sur_work(n) ||
         !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) ||
         !h2d(gs.d_tgt, tgt, (size_t) n*gs.ntf*sizeof(float)) ||
-        !h2d(gs.d_loss, &zero, sizeof(double)) || !sur_forward(n, true) ||
-        !sur_backward(n, seg, &nseg) ||
+        !h2d(gs.d_loss, &zero, sizeof(double)) ||
+        !sur_train_pass(n, seg, &nseg) ||
         fteik_k_sur_grad_adam(seg, nseg, gs.d_g, gs.d_p, gs.d_m, gs.d_v,
                               gs.d_pb, gs.np, 0, 0.0f, 0.0f, 0.0f, 0.0f,
                               0.0f, 0.0f, g_dev.stream) != 0 ||
@@ -373,7 +398,7 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
         ok = fteik_k_sur_samples(gl.d_tab, gl.ldgrd, gs.d_geo, gl.ngrd,
                                  gs.seed, pool, first, batch, gs.d_x,
                                  gs.d_tgt, gs.ntf, gs.d_loss, s) == 0 &&
-             sur_forward(batch, true) && sur_backward(batch, seg, &nseg);
+             sur_train_pass(batch, seg, &nseg);
         gs.step++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f/(1.0f - powf(b1, (float) gs.step));

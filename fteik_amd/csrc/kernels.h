@@ -204,7 +204,7 @@ int fteik_k_gemm_bf16(const void* A, long sam, long sak, const void* B,
                       long sbk, long sbn, float* C, long ldc, int M, int N,
                       int K, const float* bias, int accumulate,
                       hipStream_t stream);
-// training GEMMs with fused epilogues (k <= 256 except the partial form)
+// GEMMs with fused epilogues (k <= 256 except the partial form)
 int fteik_k_gemm_parts(int M, int N, int K);
 int fteik_k_gemm_partial(const void* A, long sam, long sak, const void* B,
                          long sbk, long sbn, int M, int N, int K, float* P,
@@ -213,27 +213,48 @@ int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
                       long sbk, long sbn, int M, int N, int K,
                       const float* bias, float* z, void* h, long ld,
                       hipStream_t stream);
-int fteik_k_gemm_silu_bwd(const void* A, long sam, long sak, const void* B,
-                          long sbk, long sbn, int M, int N, int K,
-                          const float* z, void* dz, long ld, float* colp,
-                          hipStream_t stream);
-int fteik_k_gemm_mse(const void* A, long sam, long sak, const void* B,
-                     long sbk, long sbn, int M, int N, int K,
-                     const float* bias, const float* tgt, void* dy, long ld,
-                     float inv, double* loss, float* colp, hipStream_t stream);
 int fteik_k_f2bf(const float* a, void* b, long n, hipStream_t stream);
 void fteik_k_sur_release(void);
 int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
                        float* z, void* h, long rows, int H, hipStream_t stream);
-int fteik_k_sur_dw1_parts(const void* dz, const float* x, long rows, int H,
-                          int nchunks, float* out, hipStream_t stream);
 // A range of parameters whose gradient is the in-order sum of nparts parts
 // src[z*pstride + i - off].
 struct FteikGradSeg {
     long off, len, pstride;
     const float* src;
     int nparts;
+    // weights: the segment is a row-major matrix with ncol columns whose
+    // BF16 transpose is also kept, at tcopy with row stride ldt (null:
+    // none)
+    unsigned short* tcopy;
+    int ncol, ldt;
 };
+// One fused training pass of the surrogate over 32-row blocks (forward,
+// squared error, backward through the activations); see surrogate.hip.
+struct FteikMlpArgs {
+    const float* x;              // [rows][3]
+    const float* tgt;            // [rows][T]
+    long rows;
+    int T, TP;                   // outputs, and T rounded up to 16
+    const float* p;              // FP32 parameters (W1, biases)
+    const unsigned short* pb;    // BF16 parameters, weights [out][in]
+    const unsigned short* pbT;   // BF16 transposed weights [in][out]
+    long oW1, ob1, oW[3], ob[3], oW5, ob5;   // offsets in p and pb
+    long oWT[3], oW5T;           // offsets in pbT (W5T rows padded to TP)
+    float* z[4];                 // pre-activations [rows][256]
+    unsigned short* h[4];        // activations [rows][256]
+    unsigned short* dz[4];       // pre-activation gradients (dz[0] unused)
+    unsigned short* dy;          // output gradient [rows][T]
+    float* colp;                 // bias-gradient parts: 5 slots of
+                                 // [nblk][256] (slot 3: [nblk][T])
+    float* dw1p;                 // first-layer weight-gradient parts [nblk][768]
+    float inv;                   // 1 / (rows T)
+    double* loss;
+};
+constexpr int kMlpRows = 32;     // rows per workgroup of the fused pass
+int fteik_k_mlp_train(const FteikMlpArgs* a, hipStream_t stream);
+int fteik_k_sur_transpose(const void* src, int rows, int cols, void* dst,
+                          int ldt, hipStream_t stream);
 constexpr int kSurMaxSeg = 12;
 int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
                           float* p, float* m, float* v, void* pb, long n,

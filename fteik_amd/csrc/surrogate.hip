@@ -21,6 +21,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x32 __attribute__((ext_vector_type(32)));
 typedef unsigned short bf16_t;          // storage type
 
 constexpr int TPB = 256;
@@ -36,7 +37,12 @@ __device__ __forceinline__ float bf2f(bf16_t h)
 {
     return __uint_as_float(((unsigned) h) << 16);
 }
-__device__ __forceinline__ float sigm(float z) { return 1.0f/(1.0f + __expf(-z)); }
+// logistic function with the hardware reciprocal (1 ulp; the network's
+// activations are rounded to BF16 anyway)
+__device__ __forceinline__ float sigm(float z)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float silu(float z) { return z*sigm(z); }
 __device__ __forceinline__ float dsilu(float z)
 {
@@ -72,27 +78,17 @@ constexpr int GT = 64, GK = 32, GKW = 256, LDK = GKW + 8;
 //   EP_STORE     C = acc + bias (+ C when accumulating)
 //   EP_SPLIT     P[z][m][n] = acc over the k range of split z
 //   EP_SILU      z = acc + bias into C, silu(z) as BF16 into hout
-//   EP_SILU_BWD  dz = acc * silu'(zin) as BF16 into hout, and the column
-//                sums of those BF16 values over the tile's 64 rows into
-//                colp[blockIdx.y][n] (bias gradient parts)
-//   EP_MSE       y = acc + bias, dy = 2 (y - tgt) inv as BF16 into hout, its
-//                column sums into colp, inv * sum (y - tgt)^2 added to loss
-enum { EP_STORE = 0, EP_SPLIT = 1, EP_SILU = 2, EP_SILU_BWD = 3, EP_MSE = 4 };
+enum { EP_STORE = 0, EP_SPLIT = 1, EP_SILU = 2 };
 
 struct GemmArgs {
     const bf16_t* A; long sam, sak;
     const bf16_t* B; long sbk, sbn;
     int M, N, K, kper;
-    float* C; long ldc;          // C, zin, hout and tgt share the stride ldc
+    float* C; long ldc;          // C and hout share the stride ldc
     const float* bias;
     int accumulate;
     float* P;
-    const float* zin;
     bf16_t* hout;
-    float* colp;
-    const float* tgt;
-    float inv;
-    double* loss;
 };
 
 // Thread t's 16-byte piece of an operand tile [64 rows][32 k] (rows r0..,
@@ -324,42 +320,6 @@ void k_gemm(GemmArgs g)
                 if (EP == EP_SILU) g.hout[m*g.ldc + n] = f2bf(silu(v));
             }
         }
-    } else {
-        __shared__ float csum[2][32];
-        __shared__ double lsum[4];
-        const float bn = (EP == EP_MSE && g.bias && nok) ? g.bias[n] : 0.0f;
-        float s = 0.0f;
-        double l = 0.0;
-        #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const long m = row(r);
-            if (m < g.M && nok) {
-                const long o = m*g.ldc + n;
-                bf16_t q;
-                if (EP == EP_SILU_BWD) {
-                    q = f2bf((acc[r] + bn)*dsilu(g.zin[o]));
-                } else {
-                    const float d = (acc[r] + bn) - g.tgt[o];
-                    q = f2bf(2.0f*d*g.inv);
-                    l += (double) d*d;
-                }
-                g.hout[o] = q;
-                s += bf2f(q);
-            }
-        }
-        s += __shfl_xor(s, 32);           // lanes l and l + 32: same column
-        if (wm == 1 && lane < 32) csum[wn][lane] = s;
-        if (EP == EP_MSE) {
-            #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-            if (lane == 0) lsum[w] = l;
-        }
-        __syncthreads();
-        if (wm == 0 && lane < 32 && nok)
-            g.colp[(long) by*g.N + n] = s + csum[wn][lane];
-        if (EP == EP_MSE && t == 0)
-            atomicAdd(g.loss, ((lsum[0] + lsum[1]) + (lsum[2] + lsum[3]))
-                              *(double) g.inv);
     }
 }
 
@@ -432,50 +392,13 @@ __global__ void k_layer1(const float* __restrict__ x, const float* __restrict__ 
     }
 }
 
-// Parts of the first layer's weight gradient: chunk c of `per` rows gives
-// out[c][3 j + d] = sum over its rows r of dz0[r][j] x[r][d].  Thread
-// (u, j) of the H x 4 block sums the u-th quarter of the chunk for hidden
-// unit j (a row's dz0 values are read contiguously); the quarters are
-// added in order.
-__global__ __launch_bounds__(1024)
-void k_dw1_part(const bf16_t* __restrict__ dz, const float* __restrict__ x,
-                long rows, int H, long per, float* __restrict__ out)
-{
-    __shared__ float red[3][4][256];
-    const int j = threadIdx.x, u = threadIdx.y;
-    const long c0 = (long) blockIdx.x*per, q = (per + 3)/4;
-    const long r0 = min(rows, c0 + u*q), r1 = min(rows, min(c0 + per, r0 + q));
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-    long r = r0;
-    for (; r + 8 <= r1; r += 8) {
-        float gz[8];
-        #pragma unroll
-        for (int e = 0; e < 8; e++) gz[e] = bf2f(dz[(r + e)*H + j]);
-        #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            s0 += gz[e]*x[3*(r + e)]; s1 += gz[e]*x[3*(r + e) + 1];
-            s2 += gz[e]*x[3*(r + e) + 2];
-        }
-    }
-    for (; r < r1; r++) {
-        const float gz = bf2f(dz[r*H + j]);
-        s0 += gz*x[3*r]; s1 += gz*x[3*r + 1]; s2 += gz*x[3*r + 2];
-    }
-    red[0][u][j] = s0; red[1][u][j] = s1; red[2][u][j] = s2;
-    __syncthreads();
-    if (u < 3) {
-        const float v = ((red[u][0][j] + red[u][1][j]) + red[u][2][j])
-                      + red[u][3][j];
-        out[(long) blockIdx.x*3*H + 3*j + u] = v;
-    }
-}
-
 // Parameter gradients from their parts, then (adam != 0) the Adam step.
 // Segment q covers parameters [off, off + len): gradient i is the sum of
 // src[z*pstride + i - off] over its parts z in increasing order (split-K
-// partial products, per-row-block column sums, per-chunk first-layer sums),
+// partial products, per-row-block column sums, per-block first-layer sums),
 // so the result does not depend on timing.  The FP32 master parameters are
-// updated and their BF16 copies refreshed.
+// updated and their BF16 copies refreshed, for weight segments also the
+// transposed BF16 copy.
 struct GradTable {
     FteikGradSeg s[kSurMaxSeg];
     int nseg;
@@ -488,14 +411,21 @@ __global__ void k_grad_adam(GradTable tab, float* __restrict__ g,
                             float eps, float c1, float c2)
 {
     GRID_STRIDE(i, n) {
-        int q = 0;
-        while (q + 1 < tab.nseg && i >= tab.s[q + 1].off) q++;
-        const long o = i - tab.s[q].off;
-        const float* src = tab.s[q].src;
-        const long ps = tab.s[q].pstride;
-        const int np = tab.s[q].nparts;
+        int qs = 0;
+        while (qs + 1 < tab.nseg && i >= tab.s[qs + 1].off) qs++;
+        const long o = i - tab.s[qs].off;
+        const float* src = tab.s[qs].src;
+        const long ps = tab.s[qs].pstride;
+        const int np = tab.s[qs].nparts;
         float gi = 0.0f;
         int z = 0;
+        for (; z + 32 <= np; z += 32) {
+            float u[32];
+            #pragma unroll
+            for (int e = 0; e < 32; e++) u[e] = src[(long) (z + e)*ps + o];
+            #pragma unroll
+            for (int e = 0; e < 32; e++) gi += u[e];
+        }
         for (; z + 8 <= np; z += 8) {
             float u[8];
             #pragma unroll
@@ -511,7 +441,13 @@ __global__ void k_grad_adam(GradTable tab, float* __restrict__ g,
             m[i] = mi; v[i] = vi;
             const float pi = p[i] - lr*(mi*c1)/(sqrtf(vi*c2) + eps);
             p[i] = pi;
-            pb[i] = f2bf(pi);
+            const bf16_t q = f2bf(pi);
+            pb[i] = q;
+            if (tab.s[qs].tcopy) {
+                const long rr = o / tab.s[qs].ncol;
+                const long cc = o - rr*tab.s[qs].ncol;
+                tab.s[qs].tcopy[cc*tab.s[qs].ldt + rr] = q;
+            }
         }
     }
 }
@@ -520,6 +456,291 @@ __global__ void k_f2bf(const float* __restrict__ a, bf16_t* __restrict__ b,
                        long n)
 {
     GRID_STRIDE(i, n) b[i] = f2bf(a[i]);
+}
+
+// ----------------------------------------------------- fused training pass
+// One workgroup per block of kMlpRows = 32 rows runs the whole network on
+// them: layer 1 (3 -> 256) on the VALU, then the three hidden layers and
+// the output layer as MFMA products of the block's activations (BF16, in
+// LDS) with the weights (BF16, read from L2 as 16-byte vectors), the
+// squared-error gradient, and back through the layers: dh = dz W from the
+// transposed weight copies, dz = dh silu'(z).  Every output is one MFMA
+// chain over k in steps of 16, in increasing k, as in k_gemm, so z, h, dy
+// and dz are the values the separate kernels compute.  The pass writes
+// what the weight gradients need (h_0..h_3, dz_1..dz_3, dy), the block's
+// bias-gradient column sums and its first-layer weight-gradient sums.
+constexpr int MR = kMlpRows, HU = 256, LDM = HU + 8;
+
+// The pass is a chain of stages (3 hidden layers, the output layer, the
+// output layer back, 3 hidden layers back), each an MFMA product of the
+// block's 32 rows in LDS (A) with a weight matrix (B, 8 consecutive k per
+// lane, read from L2), then an epilogue.  Loads and stores share one
+// counter of at most 63 operations in flight, and a load issued after a
+// store is waited for together with it, so:
+//   * each stage issues the next stage's weight loads and epilogue inputs
+//     right after its MFMAs, before its own stores;
+//   * results leave through LDS as whole rows, 16 bytes a thread (z, h and
+//     dz: 12 stores a thread per stage instead of 64), and the backward
+//     pre-activations arrive the same way.
+constexpr int LDZ = HU + 4;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_mlp_train(FteikMlpArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t hb[2][MR][LDM];
+    __shared__ __attribute__((aligned(16))) float zb[2][MR][LDZ];
+    __shared__ double lsum[2];
+    __shared__ float xs[MR][3];       // the block's inputs
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int blk = blockIdx.x, nblk = gridDim.x;
+    const long r0 = (long) blk*MR, rows = g.rows;
+    const int T = g.T;
+    const int lr = lane & 31, lk8 = 8*(lane >> 5);
+    auto rowof = [&](int r) { return 8*(r >> 2) + 4*(lane >> 5) + (r & 3); };
+    const int n0 = 64*w + lr, n1 = n0 + 32;       // this lane's columns
+    const int no = 32*w + lr, noc = min(no, T - 1);   // output layer
+    bf16x8 bw[2][HU/16];       // weights of the stage (two column tiles)
+    float ec[2][16], en[2][16];  // epilogue inputs: this stage, next stage
+    f32x32 zr;                 // pre-activation rows on their way to LDS
+                               // (a vector value, so it stays in registers)
+    // weights [out][in] (the forward layers): rows n0, n1 of W
+    auto load_w = [&](const bf16_t* W, long ld) {
+        const bf16_t* bp = W + (long) n0*ld + lk8;
+        #pragma unroll
+        for (int kk = 0; kk < HU/16; kk++) {
+            bw[0][kk] = *(const bf16x8*) (bp + 16*kk);
+            bw[1][kk] = *(const bf16x8*) (bp + 32*ld + 16*kk);
+        }
+    };
+    auto load_bias = [&](const float* b) { en[0][0] = b[n0]; en[1][0] = b[n1]; };
+    auto take = [&]() {
+        #pragma unroll
+        for (int c = 0; c < 2; c++)
+            #pragma unroll
+            for (int r = 0; r < 16; r++) ec[c][r] = en[c][r];
+    };
+    // whole rows of the block, 16 bytes a thread: FP32 rows of zb[b] to
+    // dst, BF16 rows of hb[b] to dst; z rows from src into zr, then zb[b]
+    auto put_z = [&](int b, float* dst) {
+        #pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int idx = t + 256*i, m = idx >> 6, c4 = (idx & 63)*4;
+            if (r0 + m < rows)
+                *(float4*) (dst + (r0 + m)*HU + c4) = *(const float4*) &zb[b][m][c4];
+        }
+    };
+    auto put_h = [&](int b, bf16_t* dst) {
+        #pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int idx = t + 256*i, m = idx >> 5, c8 = (idx & 31)*8;
+            if (r0 + m < rows)
+                *(uint4*) (dst + (r0 + m)*HU + c8) = *(const uint4*) &hb[b][m][c8];
+        }
+    };
+    auto get_z = [&](const float* src) {
+        #pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int idx = t + 256*i, m = idx >> 6, c4 = (idx & 63)*4;
+            const float4 v = *(const float4*) (src + min(r0 + m, rows - 1)*HU + c4);
+            zr[4*i] = v.x; zr[4*i + 1] = v.y; zr[4*i + 2] = v.z; zr[4*i + 3] = v.w;
+        }
+    };
+    auto park_z = [&](int b) {
+        #pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int idx = t + 256*i, m = idx >> 6, c4 = (idx & 63)*4;
+            *(float4*) &zb[b][m][c4] = make_float4(zr[4*i], zr[4*i + 1],
+                                                   zr[4*i + 2], zr[4*i + 3]);
+        }
+    };
+    // A = hb[cur], B = bw, nk steps of 16 (nk <= HU/16, wave-uniform)
+    auto product = [&](int cur, int nk, f32x16& acc0, f32x16& acc1) {
+        const bf16_t* ap = &hb[cur][lr][lk8];
+        #pragma unroll
+        for (int kk = 0; kk < HU/16; kk++) {
+            if (kk < nk) {
+                const bf16x8 a = *(const bf16x8*) (ap + 16*kk);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[0][kk],
+                                                               acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[1][kk],
+                                                               acc1, 0, 0, 0);
+            }
+        }
+    };
+    // ---- layer 1 (as k_layer1): z0 = W1 x + b1, h0 = silu(z0); the first
+    // hidden layer's weights and biases are loaded before its stores
+    if (t < MR*3) {
+        const int m = t / 3;
+        xs[m][t - 3*m] = g.x[3*min(r0 + m, rows - 1) + t - 3*m];
+    }
+    // thread t computes unit j = t of every row
+    const float w10 = g.p[g.oW1 + 3*t], w11 = g.p[g.oW1 + 3*t + 1],
+                w12 = g.p[g.oW1 + 3*t + 2], b1t = g.p[g.ob1 + t];
+    load_w(g.pb + g.oW[0], HU);
+    load_bias(g.p + g.ob[0]);
+    __syncthreads();
+    #pragma unroll 8
+    for (int m = 0; m < MR; m++) {
+        const float v = w10*xs[m][0] + w11*xs[m][1] + w12*xs[m][2] + b1t;
+        zb[1][m][t] = v;
+        hb[0][m][t] = f2bf(silu(v));
+    }
+    __syncthreads();
+    put_z(1, g.z[0]);
+    put_h(0, g.h[0]);
+    int cur = 0;
+    // ---- hidden layers: z_{l+1} = h_l W_l^T + b_l (stage staging buffer
+    // zb[l & 1])
+    #pragma unroll
+    for (int l = 0; l < 3; l++) {
+        f32x16 acc0 = {}, acc1 = {};
+        product(cur, HU/16, acc0, acc1);
+        take();
+        if (l < 2) {
+            load_w(g.pb + g.oW[l + 1], HU);
+            load_bias(g.p + g.ob[l + 1]);
+        } else {
+            // the output layer: rows no of W5 (clamped; waves 2, 3 load
+            // and multiply the last row and use nothing of it)
+            const bf16_t* bp = g.pb + g.oW5 + (long) noc*HU + lk8;
+            #pragma unroll
+            for (int kk = 0; kk < HU/16; kk++) bw[0][kk] = *(const bf16x8*) (bp + 16*kk);
+            en[0][0] = g.p[g.ob5 + noc];
+            #pragma unroll
+            for (int r = 0; r < 16; r++)
+                en[1][r] = g.tgt[min(r0 + rowof(r), rows - 1)*T + noc];
+        }
+        #pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int n = c ? n1 : n0;
+            const float bn = ec[c][0];
+            #pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int m = rowof(r);
+                const float v = (c ? acc1[r] : acc0[r]) + bn;
+                hb[cur ^ 1][m][n] = f2bf(silu(v));
+                zb[l & 1][m][n] = v;
+            }
+        }
+        __syncthreads();
+        put_z(l & 1, g.z[l + 1]);
+        put_h(cur ^ 1, g.h[l + 1]);
+        cur ^= 1;
+    }
+    // ---- output layer and dy = 2 (y - tgt) / (rows T) (waves 0 and 1,
+    // 32 columns each); dy also becomes the A operand of the way back
+    {
+        f32x16 acc = {}, unused = {};
+        {
+            const bf16_t* ap = &hb[cur][lr][lk8];
+            #pragma unroll
+            for (int kk = 0; kk < HU/16; kk++)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    *(const bf16x8*) (ap + 16*kk), bw[0][kk], acc, 0, 0, 0);
+        }
+        (void) unused;
+        take();
+        // next: the output layer back (W5 transposed, k = TP) and z3
+        {
+            const bf16_t* bp = g.pbT + g.oW5T + (long) n0*g.TP + lk8;
+            #pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int k = min(kk, g.TP/16 - 1);
+                bw[0][kk] = *(const bf16x8*) (bp + 16*k);
+                bw[1][kk] = *(const bf16x8*) (bp + 32L*g.TP + 16*k);
+            }
+        }
+        get_z(g.z[3]);
+        double l2 = 0.0;
+        if (w < 2) {
+            const float bn = ec[0][0];
+            float s = 0.0f;
+            #pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int m = rowof(r);
+                const long row = r0 + m;
+                bf16_t q = 0;
+                if (no < T && row < rows) {
+                    const float d = (acc[r] + bn) - ec[1][r];
+                    q = f2bf(2.0f*d*g.inv);
+                    g.dy[row*T + no] = q;
+                    s += bf2f(q);
+                    l2 += (double) d*d;
+                }
+                hb[cur ^ 1][m][no] = q;
+            }
+            s += __shfl_xor(s, 32);        // lanes l and l + 32: same column
+            if (lane < 32 && no < T) g.colp[3L*nblk*HU + (long) blk*T + no] = s;
+            #pragma unroll
+            for (int o = 32; o > 0; o >>= 1) l2 += __shfl_xor(l2, o);
+            if (lane == 0) lsum[w] = l2;
+        }
+        park_z(1);
+        __syncthreads();
+        if (t == 0) atomicAdd(g.loss, (lsum[0] + lsum[1])*(double) g.inv);
+        cur ^= 1;
+    }
+    // ---- the way back: dh = dz_next W (B from the transposed copies), then
+    // dz = dh silu'(z) into LDS, out as whole rows (when a weight gradient
+    // needs it) and summed over the block's rows (bias-gradient parts).
+    // Stage 0 is the output layer (k = TP), stages 1..3 the hidden layers
+    // 2, 1, 0; stage st reads z from zb[(st + 1) & 1].
+    #pragma unroll
+    for (int st = 0; st < 4; st++) {
+        const int l = 3 - st;                 // dz_l is made here
+        f32x16 acc0 = {}, acc1 = {};
+        product(cur, st == 0 ? g.TP/16 : HU/16, acc0, acc1);
+        if (l > 0) {
+            load_w(g.pbT + g.oWT[l - 1], HU);
+            get_z(g.z[l - 1]);
+        }
+        const int zi = (st + 1) & 1;
+        const int slot = (l == 3) ? 2 : (l >= 1 ? l - 1 : 4);
+        #pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int n = c ? n1 : n0;
+            float s = 0.0f;
+            #pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int m = rowof(r);
+                bf16_t q = 0;
+                if (r0 + m < rows) {
+                    q = f2bf(((c ? acc1[r] : acc0[r]) + 0.0f)*dsilu(zb[zi][m][n]));
+                    s += bf2f(q);
+                }
+                hb[cur ^ 1][m][n] = q;
+            }
+            s += __shfl_xor(s, 32);
+            if (lane < 32) g.colp[(long) slot*nblk*HU + (long) blk*HU + n] = s;
+        }
+        if (l > 0) park_z(st & 1);
+        __syncthreads();
+        if (l > 0) put_h(cur ^ 1, g.dz[l]);
+        cur ^= 1;
+    }
+    // ---- first-layer weight-gradient sums of the block (thread = unit j)
+    {
+        const int j = t;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+        const int mr = (int) min((long) MR, rows - r0);
+        for (int m = 0; m < mr; m++) {
+            const float gz = bf2f(hb[cur][m][j]);
+            s0 += gz*xs[m][0]; s1 += gz*xs[m][1]; s2 += gz*xs[m][2];
+        }
+        float* o = g.dw1p + (long) blk*3*HU + 3*j;
+        o[0] = s0; o[1] = s1; o[2] = s2;
+    }
+}
+
+// dst[c*ldt + r] = src[r*cols + c] (BF16)
+__global__ void k_transpose_bf16(const bf16_t* __restrict__ src, int rows,
+                                 int cols, bf16_t* __restrict__ dst, int ldt)
+{
+    GRID_STRIDE(i, (long) rows*cols) {
+        const long r = i / cols, c = i - r*cols;
+        dst[c*ldt + r] = src[i];
+    }
 }
 
 // ------------------------------------------------------------ data
@@ -931,29 +1152,6 @@ int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
     return launch_gemm<EP_SILU>(g, 1, s);
 }
 
-int fteik_k_gemm_silu_bwd(const void* A, long sam, long sak, const void* B,
-                          long sbk, long sbn, int M, int N, int K,
-                          const float* z, void* dz, long ld, float* colp,
-                          hipStream_t s)
-{
-    if (M < 1 || N < 1 || K < 1 || K > GKW) return 2;
-    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
-    g.zin = z; g.hout = (bf16_t*) dz; g.ldc = ld; g.colp = colp;
-    return launch_gemm<EP_SILU_BWD>(g, 1, s);
-}
-
-int fteik_k_gemm_mse(const void* A, long sam, long sak, const void* B,
-                     long sbk, long sbn, int M, int N, int K,
-                     const float* bias, const float* tgt, void* dy, long ld,
-                     float inv, double* loss, float* colp, hipStream_t s)
-{
-    if (M < 1 || N < 1 || K < 1 || K > GKW) return 2;
-    GemmArgs g = gemm_args(A, sam, sak, B, sbk, sbn, M, N, K);
-    g.bias = bias; g.tgt = tgt; g.hout = (bf16_t*) dy; g.ldc = ld;
-    g.inv = inv; g.loss = loss; g.colp = colp;
-    return launch_gemm<EP_MSE>(g, 1, s);
-}
-
 void fteik_k_sur_release(void)
 {
     if (g_split) hipFree(g_split);
@@ -976,16 +1174,6 @@ int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
     return ok();
 }
 
-int fteik_k_sur_dw1_parts(const void* dz, const float* x, long rows, int H,
-                          int nchunks, float* out, hipStream_t s)
-{
-    if (H < 1 || H > 256 || nchunks < 1) return 2;
-    const long per = (rows + nchunks - 1)/nchunks;
-    hipLaunchKernelGGL(k_dw1_part, dim3(nchunks), dim3(H, 4), 0, s,
-                       (const bf16_t*) dz, x, rows, H, per, out);
-    return ok();
-}
-
 int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
                           float* p, float* m, float* v, void* pb, long n,
                           int adam, float lr, float b1, float b2, float eps,
@@ -998,6 +1186,26 @@ int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
     hipLaunchKernelGGL(k_grad_adam, dim3(nblocks(n)), dim3(TPB), 0, s, tab,
                        g, p, m, v, (bf16_t*) pb, n, adam, lr, b1, b2, eps,
                        c1, c2);
+    return ok();
+}
+
+int fteik_k_mlp_train(const FteikMlpArgs* a, hipStream_t s)
+{
+    if (a->rows < 1 || a->T < 1 || a->T > 64 || a->TP < a->T ||
+        a->TP > 64 || (a->TP & 15) != 0)
+        return 2;
+    const long nblk = (a->rows + kMlpRows - 1)/kMlpRows;
+    hipLaunchKernelGGL(k_mlp_train, dim3((unsigned) nblk), dim3(256), 0, s,
+                       *a);
+    return ok();
+}
+
+int fteik_k_sur_transpose(const void* src, int rows, int cols, void* dst,
+                          int ldt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_transpose_bf16, dim3(nblocks((long) rows*cols)),
+                       dim3(TPB), 0, s, (const bf16_t*) src, rows, cols,
+                       (bf16_t*) dst, ldt);
     return ok();
 }
 
