@@ -36,7 +36,10 @@ struct Surrogate {
     // gradients (5 slots), first-layer sums per 32-row block
     float *d_P = nullptr, *d_colp = nullptr, *d_dw1p = nullptr;
     long capP = 0;
-    double *d_loss = nullptr;
+    // training samples of the whole pool (made once per training call)
+    float *d_poolx = nullptr, *d_pooltgt = nullptr;
+    long poolCap = 0;
+    double *d_loss = nullptr;          // two slots, alternating by step
     unsigned long long seed = 13;
     // inversion work buffers (grown on demand): event arrays sized by
     // events, the rest by rows R = events * starts
@@ -97,6 +100,7 @@ void free_surrogate()
     dfree(gs.d_dy); dfree(gs.d_P); dfree(gs.d_pbT);
     for (int l = 0; l < 3; l++) dfree(gs.d_dz[l]);
     dfree(gs.d_colp); dfree(gs.d_dw1p); dfree(gs.d_loss);
+    dfree(gs.d_poolx); dfree(gs.d_pooltgt);
     dfree(gs.d_zero);
     for (int l = 0; l < 4; l++) { dfree(gs.d_z[l]); dfree(gs.d_h[l]); }
     free_inv_work();
@@ -160,12 +164,13 @@ bool sur_forward(long rows)
                                    s) == 0;
 }
 
-// One training pass over `rows` rows of d_x against d_tgt (the mean
-// squared error over rows x ntf added to d_loss): the fused pass of
-// 32-row blocks through the network and back, then the weight-gradient
-// products as split-K parts; the segment table for k_grad_adam lists
-// every parameter gradient's parts in parameter order.
-bool sur_train_pass(long rows, FteikGradSeg *seg, int *nseg)
+// One training pass over `rows` rows, row r being row (first + r) % pool
+// of x and tgt (the mean squared error over rows x ntf added to *loss):
+// the fused pass of 32-row blocks through the network and back, then the
+// weight-gradient products as split-K parts; the segment table for
+// k_grad_adam lists every parameter gradient's parts in parameter order.
+bool sur_train_pass(long rows, const float *x, const float *tgt, long pool,
+                    long first, double *loss, FteikGradSeg *seg, int *nseg)
 {
     hipStream_t s = g_dev.stream;
     const int H = gs.H, T = gs.ntf, M = (int) rows;
@@ -182,7 +187,8 @@ bool sur_train_pass(long rows, FteikGradSeg *seg, int *nseg)
     float *P5 = gs.d_P + 3L*nsH*H*H;
     FteikMlpArgs a;
     memset(&a, 0, sizeof(a));
-    a.x = gs.d_x; a.tgt = gs.d_tgt; a.rows = rows; a.T = T; a.TP = gs.TP;
+    a.x = x; a.tgt = tgt; a.pool = pool; a.first = first;
+    a.rows = rows; a.T = T; a.TP = gs.TP;
     a.p = gs.d_p; a.pb = gs.d_pb; a.pbT = gs.d_pbT;
     a.oW1 = gs.oW1; a.ob1 = gs.ob1; a.oW5 = gs.oW5; a.ob5 = gs.ob5;
     a.oW5T = gs.oW5T;
@@ -193,14 +199,15 @@ bool sur_train_pass(long rows, FteikGradSeg *seg, int *nseg)
     for (int l = 0; l < 4; l++) { a.z[l] = gs.d_z[l]; a.h[l] = gs.d_h[l]; }
     a.dy = gs.d_dy; a.colp = gs.d_colp; a.dw1p = gs.d_dw1p;
     a.inv = 1.0f/(float) ((double) rows*T);
-    a.loss = gs.d_loss;
-    // weight gradients: dW5 = dy^T h3, dW_l = dz_{l+1}^T h_l
+    a.loss = loss;
+    // weight gradients in one launch: dW_l = dz_{l+1}^T h_l, dW5 = dy^T h3
+    FteikGemmPart parts[4];
+    for (int l = 0; l < 3; l++)
+        parts[l] = FteikGemmPart{gs.d_dz[l], 1, H, gs.d_h[l], H, 1, H, H, M,
+                                 gs.d_P + (long) l*nsH*H*H};
+    parts[3] = FteikGemmPart{gs.d_dy, 1, T, gs.d_h[3], H, 1, T, H, M, P5};
     bool ok = fteik_k_mlp_train(&a, s) == 0 &&
-              fteik_k_gemm_partial(gs.d_dy, 1, T, gs.d_h[3], H, 1, T, H, M,
-                                   P5, s) == 0;
-    for (int l = 0; l < 3 && ok; l++)
-        ok = fteik_k_gemm_partial(gs.d_dz[l], 1, H, gs.d_h[l], H, 1, H, H, M,
-                                  gs.d_P + (long) l*nsH*H*H, s) == 0;
+              fteik_k_gemm_partials(4, parts, s) == 0;
     auto colp = [&](int slot) { return gs.d_colp + slot*nrb*H; };
     int q = 0;
     auto add = [&](long off, long len, const float *src, int np, long ps,
@@ -292,7 +299,7 @@ void fteik_surrogate_initialize(int ntf, int nz, int nx, int ny, double dz,
               dalloc(&gs.d_p, (size_t) o) && dalloc(&gs.d_g, (size_t) o) &&
               dalloc(&gs.d_m, (size_t) o) && dalloc(&gs.d_v, (size_t) o) &&
               dalloc(&gs.d_pb, (size_t) o) && dalloc(&gs.d_pbT, nT) &&
-              dalloc(&gs.d_loss, 1) &&
+              dalloc(&gs.d_loss, 2) &&
               dalloc(&gs.d_zero, 64) &&
               h2d(gs.d_geo, &g, sizeof(g)) &&
               h2d(gs.d_p, p.data(), (size_t) o*sizeof(float));
@@ -355,10 +362,10 @@ void fteik_surrogate_gradients(int n, const float *pos, const float *tgt,
         !h2d(gs.d_x, pos, 3*(size_t) n*sizeof(float)) ||
         !h2d(gs.d_tgt, tgt, (size_t) n*gs.ntf*sizeof(float)) ||
         !h2d(gs.d_loss, &zero, sizeof(double)) ||
-        !sur_train_pass(n, seg, &nseg) ||
+        !sur_train_pass(n, gs.d_x, gs.d_tgt, n, 0, gs.d_loss, seg, &nseg) ||
         fteik_k_sur_grad_adam(seg, nseg, gs.d_g, gs.d_p, gs.d_m, gs.d_v,
                               gs.d_pb, gs.np, 0, 0.0f, 0.0f, 0.0f, 0.0f,
-                              0.0f, 0.0f, g_dev.stream) != 0 ||
+                              0.0f, 0.0f, nullptr, g_dev.stream) != 0 ||
         !d2h(grad, gs.d_g, (size_t) gs.np*sizeof(float)) ||
         !d2h(loss, gs.d_loss, sizeof(double))) {
         msg("fteik_surrogate_gradients: failed");
@@ -385,20 +392,36 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
     if (!sur_work(batch)) { msg("%s: out of GPU memory", who); *ierr = 1;
                             return; }
     hipStream_t s = g_dev.stream;
-    bool ok = true;
+    // the pool's samples, once: a step's rows are `batch` consecutive
+    // entries of the pool (wrapping), the same cells and values the
+    // per-step draw would give
+    if (gs.poolCap < pool) {
+        dfree(gs.d_poolx); dfree(gs.d_pooltgt);
+        gs.poolCap = 0;
+        if (!dalloc(&gs.d_poolx, 3*(size_t) pool) ||
+            !dalloc(&gs.d_pooltgt, (size_t) pool*gs.ntf)) {
+            dfree(gs.d_poolx); dfree(gs.d_pooltgt);
+            msg("%s: out of GPU memory", who);
+            *ierr = 1;
+            return;
+        }
+        gs.poolCap = pool;
+    }
+    bool ok = fteik_k_sur_samples(gl.d_tab, gl.ldgrd, gs.d_geo, gl.ngrd,
+                                  gs.seed, pool, 0, pool, gs.d_poolx,
+                                  gs.d_pooltgt, gs.ntf, nullptr, s) == 0 &&
+              hipMemsetAsync(gs.d_loss, 0, 2*sizeof(double), s) == hipSuccess;
     double tail = 0.0;
     const int tailN = std::max(1, std::min(steps, 50));
     FteikGradSeg seg[kSurMaxSeg];
     int nseg = 0;
     for (int it = 0; it < steps && ok; it++) {
-        const long first = (long) it*batch;
-        // the sample kernel clears the loss on the stream (no host
-        // synchronization per step, so the host queues the next steps
-        // while the GPU works)
-        ok = fteik_k_sur_samples(gl.d_tab, gl.ldgrd, gs.d_geo, gl.ngrd,
-                                 gs.seed, pool, first, batch, gs.d_x,
-                                 gs.d_tgt, gs.ntf, gs.d_loss, s) == 0 &&
-             sur_train_pass(batch, seg, &nseg);
+        // step it accumulates its loss in slot it & 1; its Adam launch
+        // clears the other slot for the next step (no host synchronization
+        // per step, so the host queues the next steps while the GPU works)
+        double *const loss = gs.d_loss + (it & 1);
+        ok = sur_train_pass(batch, gs.d_poolx, gs.d_pooltgt, pool,
+                            ((long) it*batch) % pool, loss, seg, &nseg);
         gs.step++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f/(1.0f - powf(b1, (float) gs.step));
@@ -406,10 +429,11 @@ void fteik_surrogate_train(int steps, int batch, double lr, int pool,
         const float lrt = (float) (lr*0.5*(1.0 + cos(3.14159265358979*it/steps)));
         ok = ok && fteik_k_sur_grad_adam(seg, nseg, gs.d_g, gs.d_p, gs.d_m,
                                          gs.d_v, gs.d_pb, gs.np, 1, lrt, b1,
-                                         b2, 1e-8f, c1, c2, s) == 0;
+                                         b2, 1e-8f, c1, c2,
+                                         gs.d_loss + ((it + 1) & 1), s) == 0;
         if (ok && it >= steps - tailN) {
             double l = 0.0;
-            ok = d2h(&l, gs.d_loss, sizeof(double));
+            ok = d2h(&l, loss, sizeof(double));
             tail += l;
         }
     }

@@ -209,6 +209,16 @@ int fteik_k_gemm_parts(int M, int N, int K);
 int fteik_k_gemm_partial(const void* A, long sam, long sak, const void* B,
                          long sbk, long sbn, int M, int N, int K, float* P,
                          hipStream_t stream);
+// split-K partial products of up to 4 GEMMs in one launch (each into its
+// own P, fteik_k_gemm_parts(M, N, K) parts of M x N)
+struct FteikGemmPart {
+    const void* A; long sam, sak;
+    const void* B; long sbk, sbn;
+    int M, N, K;
+    float* P;
+};
+int fteik_k_gemm_partials(int n, const FteikGemmPart* parts,
+                          hipStream_t stream);
 int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
                       long sbk, long sbn, int M, int N, int K,
                       const float* bias, float* z, void* h, long ld,
@@ -232,8 +242,9 @@ struct FteikGradSeg {
 // One fused training pass of the surrogate over 32-row blocks (forward,
 // squared error, backward through the activations); see surrogate.hip.
 struct FteikMlpArgs {
-    const float* x;              // [rows][3]
-    const float* tgt;            // [rows][T]
+    const float* x;              // [pool][3]
+    const float* tgt;            // [pool][T]
+    long pool, first;            // row r of the pass is row (first + r) % pool
     long rows;
     int T, TP;                   // outputs, and T rounded up to 16
     const float* p;              // FP32 parameters (W1, biases)
@@ -259,7 +270,8 @@ constexpr int kSurMaxSeg = 12;
 int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
                           float* p, float* m, float* v, void* pb, long n,
                           int adam, float lr, float b1, float b2, float eps,
-                          float c1, float c2, hipStream_t stream);
+                          float c1, float c2, double* lclear,
+                          hipStream_t stream);
 int fteik_k_sur_samples(const float* tab, long ldgrd, const void* geo,
                         long ngrd, unsigned long long seed, long pool,
                         long first, long rows, float* x, float* tgt, int ldt,

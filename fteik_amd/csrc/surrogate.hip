@@ -210,15 +210,13 @@ __device__ __forceinline__ void gemm_tile(int& bx, int& by, int& bz)
 // One workgroup: the 64 x 64 tile (by, bx) of C over the k range of split
 // bz (kper values each).
 template <int EP>
-__global__ __launch_bounds__(256)
-void k_gemm(GemmArgs g)
+__device__ __forceinline__ void gemm_body(const GemmArgs& g, int bx, int by,
+                                          int bz)
 {
     __shared__ __attribute__((aligned(16))) bf16_t As[GT][LDK];
     __shared__ __attribute__((aligned(16))) bf16_t Bs[GT][LDK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w & 1, wn = w >> 1;
-    int bx, by, bz;
-    gemm_tile(bx, by, bz);
     const long m0 = (long) by*GT, n0 = (long) bx*GT;
     const int kb = bz*g.kper, ke = min(g.K, kb + g.kper);
     constexpr int NCH = GKW/GK;
@@ -323,6 +321,37 @@ void k_gemm(GemmArgs g)
     }
 }
 
+template <int EP>
+__global__ __launch_bounds__(256)
+void k_gemm(GemmArgs g)
+{
+    int bx, by, bz;
+    gemm_tile(bx, by, bz);
+    gemm_body<EP>(g, bx, by, bz);
+}
+
+// Several split-K partial products in one launch: group q owns workgroups
+// [start[q], start[q + 1]) of the linear id (after the XCD dealing of
+// gemm_tile), laid out as its (column tile, row block, split) grid.
+constexpr int kMaxGroup = 4;
+struct GemmGroup {
+    GemmArgs g[kMaxGroup];
+    int nx[kMaxGroup], ny[kMaxGroup], start[kMaxGroup + 1];
+    int ng;
+};
+
+__global__ __launch_bounds__(256)
+void k_gemm_grouped(GemmGroup G)
+{
+    const int T = G.start[G.ng];
+    int L = blockIdx.x;
+    if ((T & 7) == 0) L = (L & 7)*(T >> 3) + (L >> 3);
+    int q = 0;
+    while (q + 1 < G.ng && L >= G.start[q + 1]) q++;
+    const int loc = L - G.start[q], nx = G.nx[q], ny = G.ny[q];
+    gemm_body<EP_SPLIT>(G.g[q], loc % nx, (loc / nx) % ny, loc / (nx*ny));
+}
+
 // C = sum of the nsplit partial products (in split order) + bias.
 __global__ void k_gemm_reduce(const float* __restrict__ P, int nsplit, int M,
                               int N, float* __restrict__ C, long ldc,
@@ -408,8 +437,10 @@ __global__ void k_grad_adam(GradTable tab, float* __restrict__ g,
                             float* __restrict__ p, float* __restrict__ m,
                             float* __restrict__ v, bf16_t* __restrict__ pb,
                             long n, int adam, float lr, float b1, float b2,
-                            float eps, float c1, float c2)
+                            float eps, float c1, float c2,
+                            double* __restrict__ lclear)
 {
+    if (lclear && blockIdx.x == 0 && threadIdx.x == 0) *lclear = 0.0;
     GRID_STRIDE(i, n) {
         int qs = 0;
         while (qs + 1 < tab.nseg && i >= tab.s[qs + 1].off) qs++;
@@ -570,9 +601,11 @@ void k_mlp_train(FteikMlpArgs g)
     };
     // ---- layer 1 (as k_layer1): z0 = W1 x + b1, h0 = silu(z0); the first
     // hidden layer's weights and biases are loaded before its stores
+    // row r of the pass is row (first + r) % pool of x and tgt
+    auto prow = [&](long r) { return (g.first + r) % g.pool; };
     if (t < MR*3) {
         const int m = t / 3;
-        xs[m][t - 3*m] = g.x[3*min(r0 + m, rows - 1) + t - 3*m];
+        xs[m][t - 3*m] = g.x[3*prow(min(r0 + m, rows - 1)) + t - 3*m];
     }
     // thread t computes unit j = t of every row
     const float w10 = g.p[g.oW1 + 3*t], w11 = g.p[g.oW1 + 3*t + 1],
@@ -609,7 +642,7 @@ void k_mlp_train(FteikMlpArgs g)
             en[0][0] = g.p[g.ob5 + noc];
             #pragma unroll
             for (int r = 0; r < 16; r++)
-                en[1][r] = g.tgt[min(r0 + rowof(r), rows - 1)*T + noc];
+                en[1][r] = g.tgt[prow(min(r0 + rowof(r), rows - 1))*T + noc];
         }
         #pragma unroll
         for (int c = 0; c < 2; c++) {
@@ -1141,6 +1174,27 @@ int fteik_k_gemm_partial(const void* A, long sam, long sak, const void* B,
     return launch_gemm<EP_SPLIT>(g, nsplit, s);
 }
 
+int fteik_k_gemm_partials(int n, const FteikGemmPart* parts, hipStream_t s)
+{
+    if (n < 1 || n > kMaxGroup) return 2;
+    GemmGroup G = {};
+    G.ng = n;
+    G.start[0] = 0;
+    for (int q = 0; q < n; q++) {
+        const FteikGemmPart& p = parts[q];
+        if (p.M < 1 || p.N < 1 || p.K < 1) return 2;
+        G.g[q] = gemm_args(p.A, p.sam, p.sak, p.B, p.sbk, p.sbn, p.M, p.N,
+                           p.K);
+        const int nsplit = gemm_split(p.M, p.N, p.K, &G.g[q].kper);
+        G.g[q].P = p.P;
+        G.nx[q] = (p.N + GT - 1)/GT;
+        G.ny[q] = (p.M + GT - 1)/GT;
+        G.start[q + 1] = G.start[q] + G.nx[q]*G.ny[q]*nsplit;
+    }
+    hipLaunchKernelGGL(k_gemm_grouped, dim3(G.start[n]), dim3(256), 0, s, G);
+    return ok();
+}
+
 int fteik_k_gemm_silu(const void* A, long sam, long sak, const void* B,
                       long sbk, long sbn, int M, int N, int K,
                       const float* bias, float* z, void* h, long ld,
@@ -1177,7 +1231,7 @@ int fteik_k_sur_layer1(const float* x, const float* W1, const float* b1,
 int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
                           float* p, float* m, float* v, void* pb, long n,
                           int adam, float lr, float b1, float b2, float eps,
-                          float c1, float c2, hipStream_t s)
+                          float c1, float c2, double* lclear, hipStream_t s)
 {
     if (nseg < 1 || nseg > kSurMaxSeg) return 2;
     GradTable tab = {};
@@ -1185,7 +1239,7 @@ int fteik_k_sur_grad_adam(const FteikGradSeg* segs, int nseg, float* g,
     tab.nseg = nseg;
     hipLaunchKernelGGL(k_grad_adam, dim3(nblocks(n)), dim3(TPB), 0, s, tab,
                        g, p, m, v, (bf16_t*) pb, n, adam, lr, b1, b2, eps,
-                       c1, c2);
+                       c1, c2, lclear);
     return ok();
 }
 
