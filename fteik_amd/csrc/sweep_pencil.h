@@ -1204,7 +1204,40 @@ __device__ __forceinline__ void pencil_mb_body(const SweepArgs<R>& a,
             }
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             const int fi = t - fj - fk;
-            if (colE && fi >= ez0 && fi <= a.fz1) {
+            if constexpr (SW && !INIT) {
+                // Iteration sweeps with a store wave: every lane runs the
+                // stencil at every step, and the lanes whose node is not
+                // updated keep their old value, so the step has no branch.
+                // The ring values of lanes outside the box rows [ez0, fz1]
+                // or columns are read only by lanes that are outside too
+                // (the readers of slot t are the node's own column and its
+                // x, y and xy successors at steps t + 1 .. t + 3, whose
+                // flipped row is the same or one more), and the store wave
+                // reads in-box nodes only.
+                R v = tc[q];
+                if constexpr (STAGE) v = tbuf[threadIdx.x*HB + q];
+                const int s1 = (t - 1) & (RING - 1), s2 = (t - 2) & (RING - 1),
+                          s3 = (t - 3) & (RING - 1);
+                Nbr<R> nb;
+                nb.tv   = ring[s1][me];
+                nb.te   = ring[s1][cW];
+                nb.tn   = ring[s1][cS];
+                nb.tev  = ring[s2][cW];
+                nb.ten  = ring[s2][cSW];
+                nb.tnv  = ring[s2][cS];
+                nb.tnve = ring[s3][cSW];
+                Slw<R> sl;
+                if constexpr (STAGE) {
+                    sl = sls;
+                } else {
+                    slowness7_ud<R>(cc[q], cc[q + 1], sl);
+                }
+                const R vn = noinit3d(c, nb, sl, v);
+                const bool upd = colB && fi >= a.fz0 && fi <= a.fz1;
+                dmax = upd ? rmax(dmax, v - vn) : dmax;
+                v = upd ? vn : v;
+                ring[t & (RING - 1)][me] = v;
+            } else if (colE && fi >= ez0 && fi <= a.fz1) {
                 R v = tc[q];
                 if constexpr (STAGE) v = tbuf[threadIdx.x*HB + q];
                 if (colB && fi >= a.fz0) {
