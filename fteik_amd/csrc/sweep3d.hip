@@ -112,15 +112,18 @@ void sweep3d_pencil_mb_kernel(SweepArgs<R> a)
         pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB, ST>(
             a, dyn_of(a), a.plan[1 + blockIdx.x]);
     } else {
+        const int wg = tile_of_rank((int) blockIdx.x, a.nbx, a.nby);
+        if (wg < 0) return;
         pencil_mb_body<R, INIT, BX, BY, HB, MODE, SKIP, false, KB, ST>(
-            a, dyn_of(a), blockIdx.x);
+            a, dyn_of(a), wg);
     }
 }
 
 // A chain of nq consecutive iteration sweeps (directions dir0 + q, q = 0..
 // nq-1, cyclically; serials a.serial + q) in one launch: workgroup
-// q*ntile + tile sweeps that tile in sweep q, overlapping sweep q - 1 as
-// far as the data dependences between them allow (see the CHAIN body).
+// q*ntile + k sweeps the tile of rank k (tile_of_rank) in sweep q,
+// overlapping sweep q - 1 as far as the data dependences between them allow
+// (see the CHAIN body).
 // The iteration-sweep box is the same in every direction in flipped
 // coordinates, so the tiles are too.  The launch need not be resident at
 // once: a workgroup waits only on lower-numbered ones (its sweep's upwind
@@ -134,7 +137,8 @@ void sweep3d_chain_kernel(SweepArgs<R> a, int dir0)
 {
     const int ntile = a.nbx*a.nby;
     const int q = (int) blockIdx.x / ntile;
-    const int wg = (int) blockIdx.x - q*ntile;
+    const int wg = tile_of_rank((int) blockIdx.x - q*ntile, a.nbx, a.nby);
+    if (wg < 0) return;
     pencil_mb_body<R, false, BX, BY, HB, 2, false, true>(
         a, chain_dyn(dir0, q, a.serial), wg);
 }
@@ -147,7 +151,7 @@ void sweep3d_chain_kernel(SweepArgs<R> a, int dir0)
 // output.  Unrolling "and whose upwind tiles are skipped" gives: a tile is
 // processed iff some tile of its upwind quadrant (fbx' <= fbx and
 // fby' <= fby) has a changed column -- a 2D prefix-OR over the tiles.
-// Writes the plan (count, processed tiles in tile order, skip flags) and
+// Writes the plan (count, processed tiles in dispatch order, skip flags) and
 // stamps every tile's lastproc with this launch's serial.
 constexpr int kPlanMaxTiles = 4096;
 
@@ -207,16 +211,21 @@ void skip_plan_kernel(unsigned long long* __restrict__ lastproc,
     }
     __syncthreads();
     if (threadIdx.x < 64) {
-        // stable compaction of the processed tiles, 64 at a time
+        // compaction of the processed tiles in the dispatch order of
+        // tile_of_rank (by diagonal fbx + fby, then fbx), 64 at a time
         const int lane = threadIdx.x;
         int base = 0;
-        for (int t0 = 0; t0 < nt; t0 += 64) {
-            const int t = t0 + lane;
-            const bool run = t < nt && s_dirty[t];
-            const unsigned long long m = __ballot(run);
-            if (run)
-                plan[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = t;
-            base += __popcll(m);
+        for (int dg = 0; dg < nbx + nby - 1; dg++) {
+            const int lo = max(0, dg - (nby - 1)), hi = min(dg, nbx - 1);
+            for (int p0 = lo; p0 <= hi; p0 += 64) {
+                const int p = p0 + lane;
+                const int t = p*nby + (dg - p);
+                const bool run = (p <= hi) ? (s_dirty[t] != 0) : false;
+                const unsigned long long m = __ballot(run);
+                if (run)
+                    plan[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+                base += __popcll(m);
+            }
         }
         if (lane == 0) plan[0] = base;
     }

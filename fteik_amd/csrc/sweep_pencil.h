@@ -96,6 +96,27 @@ __device__ __forceinline__ int unflip(int f, int n, int sgn)
     return (sgn > 0) ? f + 1 : n - f;
 }
 
+// Tile (fbx*nby + fby) swept by the k-th workgroup of a launch over an
+// nbx x nby plane of tiles: tiles in order of their diagonal fbx + fby, then
+// of fbx.  A sweep starts at tile (0, 0) and tile (fbx, fby) can start about
+// one tile width of levels after its upwind neighbours (fbx - 1, fby) and
+// (fbx, fby - 1), which come earlier in this order.  When not every
+// workgroup of a launch is resident, workgroups are dispatched in order, so
+// the tiles are dispatched about when their inputs arrive; in row-major
+// order the last rows of tiles would wait for slots freed by the first and
+// lengthen the sweep.  The loop is uniform (scalar) and runs once per
+// workgroup.  -1 when k is not a tile rank (k >= nbx*nby).
+__device__ __forceinline__ int tile_of_rank(int k, int nbx, int nby)
+{
+    for (int d = 0; d < nbx + nby - 1; d++) {
+        const int lo = max(0, d - (nby - 1)), hi = min(d, nbx - 1);
+        const int n = hi - lo + 1;
+        if (k < n) return (lo + k)*nby + (d - lo - k);
+        k -= n;
+    }
+    return -1;
+}
+
 // ------------------------------------------------------- pencil sweep
 template <typename R>
 struct SweepArgs {
