@@ -119,9 +119,9 @@ void fteik_solver3d_setMixedParameters(double bias, int maxCorrections,
  * fteik_solver3d_autotune, or from a measured preference list at
  * initialize).  Flag tiles (mode 0) are accepted only when all their
  * workgroups can be resident; mailbox tiles (modes 1, 2) also launch
- * without co-residency: a tile waits only on lower-numbered tiles and
- * workgroups are dispatched in order, so the lowest unfinished tile is
- * always resident.  Every wait has a spin limit; a sweep that exceeds it
+ * without co-residency: tiles are dealt to workgroups so that a tile waits
+ * only on tiles of lower-numbered workgroups, and workgroups are dispatched
+ * in order, so the lowest unfinished tile is always resident.  Every wait has a spin limit; a sweep that exceeds it
  * fails the solve with ierr = 1 rather than hanging. */
 void fteik_solver3d_setTile(int tile, int tile32, int *ierr);
 void fteik_solver3d_getTile(int *tile, int *tile32, int *ntiles, int *ierr);
