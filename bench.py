@@ -426,10 +426,11 @@ def locator_section(L, args, nev=10000, cpu_events=16):
            "unit": "events/s", "tables_ms": ms_tab,
            "achieved_TFLOPs": flops / tgpu / 1e12, "fp32_peak_TFLOPs": 157.3,
            # the search issues unfused, unpacked FP32 operations (bitwise
-           # parity): one per lane per cycle, 16 lanes per SIMD, 4 SIMDs
-           # per CU, 256 CUs at 2.4 GHz
-           "valu_issue_peak_Tops": 39.3,
-           "valu_issue_frac": flops / tgpu / 39.3e12,
+           # parity): a wave64 FP32 instruction issues in 2 cycles (32 lanes
+           # per SIMD per cycle, the rate of the FP32 peak counted as FMAs),
+           # 4 SIMDs per CU, 256 CUs at 2.4 GHz
+           "valu_issue_peak_Tops": 78.6,
+           "valu_issue_frac": flops / tgpu / 78.6e12,
            "argmin_equals_true_cell": float(np.mean(
                np.abs(opt - 1 - hypo) == 0))}
     # CPU baseline and parity on a sample of events (oracle, fresh state)

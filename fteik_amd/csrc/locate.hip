@@ -19,6 +19,7 @@ namespace {
 
 constexpr int TL = 256;
 constexpr int MAXT = 64;      // tables held in registers
+constexpr int CH = 16;        // stations per chunk of scalar operands
 
 __device__ __forceinline__ unsigned long long make_key(float obj, int cell)
 {
@@ -117,16 +118,20 @@ void k_locate_one(const float* __restrict__ tab, int ntf, int ngrd, long ldgrd,
 // reference's sums over the picks it keeps without a branch per pick.
 // wt2 = sqrt(1/2) * wt, the product the reference forms per pick.  Each
 // wave posts its minimum key with an atomic (no workgroup barrier per
-// event).
+// event).  blockIdx.y selects a group of epg consecutive events, so the
+// launch can be cut into workgroups that fill whole rounds of the GPU
+// (event_groups).
 template <int NTF, int CPT>
 __global__ __launch_bounds__(TL)
 void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
                     const float* __restrict__ obs, const float* __restrict__ wt,
                     const float* __restrict__ wt2,
                     const float* __restrict__ xnorm, int lntf, int nev,
-                    unsigned long long* __restrict__ best)
+                    int epg, unsigned long long* __restrict__ best)
 {
     const long base = ((long) blockIdx.x*TL*CPT) + threadIdx.x;
+    // events [e0, e1) of this workgroup's event group
+    const int e0 = (int) blockIdx.y*epg, e1 = min(nev, e0 + epg);
     float T[CPT][NTF];
     #pragma unroll
     for (int q = 0; q < CPT; q++) {
@@ -135,7 +140,7 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
         for (int i = 0; i < NTF; i++)
             T[q][i] = (c < ngrd) ? tab[(long) i*ldgrd + c] : 0.0f;
     }
-    for (int e = 0; e < nev; e++) {
+    for (int e = e0; e < e1; e++) {
         const float* o = obs + (long) e*lntf;
         const float* w = wt + (long) e*lntf;
         const float* w2 = wt2 + (long) e*lntf;
@@ -143,33 +148,61 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
         float t0[CPT], ob[CPT];
         #pragma unroll
         for (int q = 0; q < CPT; q++) { t0[q] = 0.0f; ob[q] = 0.0f; }
-        #pragma unroll
-        for (int i = 0; i < NTF; i++) {
-            const float oi = o[i], wi = w[i];
+        // The event's pick times and weights are scalar operands, loaded
+        // CH stations at a time, one chunk ahead of the chunk in use: all
+        // 3*NTF at once would not fit the scalar registers, and the values
+        // spilled to vector lanes would cost a vector instruction per use.
+        // The scheduling barriers keep the loads where they are placed.
+        // The operations per (cell, station) and their order are those of
+        // the reference.
+        float cv[2][CH], cw[2][CH];
+        auto fetch = [&](int pass, int c, int b) {
+            const float* wsrc = pass ? w2 : w;
             #pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                float res = xn*(oi - T[q][i]);
-                t0[q] = t0[q] + wi*res;
+            for (int j = 0; j < CH; j++) {
+                if (c*CH + j < NTF) {
+                    cv[b][j] = o[c*CH + j];
+                    cw[b][j] = wsrc[c*CH + j];
+                }
             }
-        }
+        };
+        constexpr int NCH = (NTF + CH - 1)/CH;
+        fetch(0, 0, 0);
         #pragma unroll
-        for (int i = 0; i < NTF; i++) {
-            const float oi = o[i], wi = w2[i];
+        for (int k = 0; k < 2*NCH; k++) {
+            const int pass = k / NCH, c = k - pass*NCH, b = k & 1;
+            // the memory clobber keeps pass 2 from reusing pass 1's loads
+            // of the pick times (which would keep all of them live)
+            asm volatile("" ::: "memory");
+            if (k + 1 < 2*NCH) fetch((k + 1)/NCH, (k + 1) % NCH, b ^ 1);
+            __builtin_amdgcn_sched_barrier(0);
             #pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                float est = T[q][i] + t0[q];
-                float res = wi*(oi - est);
-                ob[q] = ob[q] + res*res;
+            for (int j = 0; j < CH; j++) {
+                const int i = c*CH + j;
+                if (i >= NTF) break;
+                const float oi = cv[b][j], wi = cw[b][j];
+                #pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    if (pass == 0) {
+                        float res = xn*(oi - T[q][i]);
+                        t0[q] = t0[q] + wi*res;
+                    } else {
+                        float est = T[q][i] + t0[q];
+                        float res = wi*(oi - est);
+                        ob[q] = ob[q] + res*res;
+                    }
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
+        // selects, not a branch per cell: under a branch the compiler sinks
+        // the cell's whole misfit computation into it
         unsigned long long key = ~0ull;
         #pragma unroll
         for (int q = 0; q < CPT; q++) {
             const long c = base + (long) q*TL;
-            if (c < ngrd) {
-                unsigned long long k = make_key(ob[q], (int) c);
-                key = (k < key) ? k : key;
-            }
+            const unsigned long long k = make_key(ob[q], (int) c);
+            key = (c < ngrd && k < key) ? k : key;
         }
         key = wave_min(key);
         if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(best + e, key);
@@ -202,6 +235,35 @@ __global__ void k_t0_at(float* __restrict__ t0, const float* __restrict__ tab,
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
+// Event groups of a batched search.  Each workgroup searches its cells for
+// the events of one group, so a launch is nblk x groups workgroups of equal
+// length.  They run in rounds of as many as are resident at once, and a
+// last round that is only partly full leaves CUs idle: 2734 cell blocks of
+// 48 tables of 161 x 161 x 81 make 3.56 rounds of 768, 89 % of the GPU
+// busy on average.  The group count minimizes rounds x events per group
+// (each group reloads its cells' tables, so groups keep at least 256
+// events).
+int event_groups(const void* fn, int nblk, int nev)
+{
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TL, 0)
+            != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) return 1;
+    const long cap = (long) per_cu*ncu;
+    if (cap <= 0 || nblk <= 0 || nev <= 0) return 1;
+    int best = 1;
+    double bt = 0.0;
+    for (int g = 1; g <= 16; g++) {
+        const long epg = (nev + g - 1)/g;
+        if (g > 1 && epg < 256) break;
+        const long rounds = ((long) nblk*g + cap - 1)/cap;
+        const double t = (double) rounds*(double) epg;
+        if (g == 1 || t < 0.999*bt) { bt = t; best = g; }
+    }
+    return best;
+}
+
 } // namespace
 
 extern "C" {
@@ -232,18 +294,24 @@ int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
                          unsigned long long* best, hipStream_t s)
 {
     // cells per thread; measured on 48 tables of 161x161x81 and 10,000
-    // events: 2 -> 29.8k, 3 -> 31.8k, 4 -> 27.9k events/s (4 costs occupancy)
+    // events: 2 -> 58.7k, 3 -> 61.7k, 4 -> 53.6k events/s (4 costs occupancy)
     constexpr int CPT = 3;
-    const int nblk = (int) ((ngrd + (long) TL*CPT - 1)/((long) TL*CPT));
-#define LB(N) case N: hipLaunchKernelGGL((k_locate_batch<N, CPT>), dim3(nblk), \
-        dim3(TL), 0, s, tables, ngrd, ldgrd, obs, wt, wt2, xnorm, lntf, nev, \
-        best); break;
+    int nblk = (int) ((ngrd + (long) TL*CPT - 1)/((long) TL*CPT));
+    const void* fn = nullptr;
+#define LB(N) case N: fn = (const void*) k_locate_batch<N, CPT>; break;
     switch (ntf) {
         LB(1) LB(2) LB(3) LB(4) LB(5) LB(6) LB(7) LB(8) LB(10) LB(12) LB(16)
         LB(20) LB(24) LB(32) LB(40) LB(48) LB(56) LB(64)
         default: return 2;
     }
 #undef LB
+    int groups = event_groups(fn, nblk, nev);
+    int epg = (nev + groups - 1)/groups;
+    void* args[] = {(void*) &tables, (void*) &ngrd, (void*) &ldgrd,
+                    (void*) &obs, (void*) &wt, (void*) &wt2, (void*) &xnorm,
+                    (void*) &lntf, (void*) &nev, (void*) &epg, (void*) &best};
+    if (hipLaunchKernel(fn, dim3(nblk, groups), dim3(TL), args, 0, s)
+            != hipSuccess) return 1;
     return ok();
 }
 
