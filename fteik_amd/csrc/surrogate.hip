@@ -879,60 +879,82 @@ __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
 // the GEMM's), then h = silu(z + b) for the primal block and
 // h_d = silu'(z_0 + b) z_d for the tangents, rounded to BF16: what
 // k_gemm followed by k_inv_act computes, without the FP32 round trip
-// through memory.  One workgroup per 32 rows; wave d computes block d with
-// its A rows held in registers over the whole K, the weights stream from
-// L2 as 16-byte vectors, and the primal wave hands silu' to the tangent
-// waves through LDS one 32-column tile at a time.
+// through memory.  Persistent: each workgroup copies W into LDS once (rows
+// padded to 264 elements so the 32 rows an MFMA operand load touches fall
+// in different banks) and then takes 32-row blocks in turn; wave d computes
+// block d with its A rows held in registers over the whole K, the next
+// block's rows loading while the current one computes, and the primal wave
+// hands silu' to the tangent waves through LDS one 32-column tile at a
+// time.  Streaming W from L2 for every wave instead read 1.3 GB of L2 per
+// launch at 80,000 rows.
+constexpr int kInvWStride = 264;
 __global__ __launch_bounds__(256)
 void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
                   const bf16_t* __restrict__ W, const float* __restrict__ b,
                   bf16_t* __restrict__ hout)
 {
     constexpr int H = 256, KS = H/16;
+    __shared__ __attribute__((aligned(16))) bf16_t ws[H*kInvWStride];
     __shared__ float ds[2][32][33];
     const int lane = threadIdx.x & 63, d = threadIdx.x >> 6;
-    const long s0 = (long) blockIdx.x*32;
-    const long arow = s0 + (lane & 31);
-    const bool aok = arow < R;
-    const bf16_t* ap = hin + ((long) d*R + (aok ? arow : 0))*H + 8*(lane >> 5);
-    bf16x8 a[KS];
-    #pragma unroll
-    for (int kk = 0; kk < KS; kk++) {
-        bf16x8 v = {};
-        if (aok) v = *(const bf16x8*) (ap + 16*kk);
-        a[kk] = v;
+    for (int c = threadIdx.x; c < H*H/8; c += 256) {
+        const int n = c >> 5, k8 = (c & 31)*8;
+        *(bf16x8*) (ws + n*kInvWStride + k8) = *(const bf16x8*) (W + n*H + k8);
     }
-    for (int ct = 0; ct < H/32; ct++) {
-        const int n = 32*ct + (lane & 31);
-        const bf16_t* wp = W + (long) n*H + 8*(lane >> 5);
-        f32x16 acc = {};
+    const long nrb = (R + 31)/32;
+    auto load_a = [&](long rb, bf16x8* a) {
+        const long arow = rb*32 + (lane & 31);
+        const bool aok = rb < nrb && arow < R;
+        const bf16_t* ap = hin + ((long) d*R + (aok ? arow : 0))*H
+                         + 8*(lane >> 5);
         #pragma unroll
         for (int kk = 0; kk < KS; kk++) {
-            const bf16x8 bv = *(const bf16x8*) (wp + 16*kk);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk], bv, acc,
-                                                          0, 0, 0);
+            bf16x8 v = {};
+            if (aok) v = *(const bf16x8*) (ap + 16*kk);
+            a[kk] = v;
         }
-        const float bn = b[n];
-        float (*dsb)[33] = ds[ct & 1];
-        if (d == 0) {
+    };
+    bf16x8 a[KS], an[KS];
+    long rb = blockIdx.x;
+    load_a(rb, a);
+    __syncthreads();
+    for (; rb < nrb; rb += gridDim.x) {
+        load_a(rb + gridDim.x, an);
+        const long s0 = rb*32;
+        for (int ct = 0; ct < H/32; ct++) {
+            const int n = 32*ct + (lane & 31);
+            const bf16_t* wp = ws + n*kInvWStride + 8*(lane >> 5);
+            f32x16 acc = {};
             #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
-                const float z = acc[r] + bn;
-                dsb[m][lane & 31] = dsilu(z);
-                if (s0 + m < R) hout[(s0 + m)*H + n] = f2bf(silu(z));
+            for (int kk = 0; kk < KS; kk++) {
+                const bf16x8 bv = *(const bf16x8*) (wp + 16*kk);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk], bv, acc,
+                                                              0, 0, 0);
+            }
+            const float bn = b[n];
+            float (*dsb)[33] = ds[ct & 1];
+            if (d == 0) {
+                #pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+                    const float z = acc[r] + bn;
+                    dsb[m][lane & 31] = dsilu(z);
+                    if (s0 + m < R) hout[(s0 + m)*H + n] = f2bf(silu(z));
+                }
+            }
+            __syncthreads();
+            if (d > 0) {
+                #pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
+                    if (s0 + m < R)
+                        hout[((long) d*R + s0 + m)*H + n] =
+                            f2bf(dsb[m][lane & 31]*acc[r]);
+                }
             }
         }
-        __syncthreads();
-        if (d > 0) {
-            #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
-                if (s0 + m < R)
-                    hout[((long) d*R + s0 + m)*H + n] =
-                        f2bf(dsb[m][lane & 31]*acc[r]);
-            }
-        }
+        #pragma unroll
+        for (int kk = 0; kk < KS; kk++) a[kk] = an[kk];
     }
 }
 
@@ -1287,9 +1309,19 @@ int fteik_k_inv_hidden(const void* hin, long R, const void* W,
                        const float* b, void* hout, int H, hipStream_t s)
 {
     if (H != 256 || R < 1) return 2;
-    hipLaunchKernelGGL(k_inv_hidden, dim3((unsigned) ((R + 31)/32)),
-                       dim3(256), 0, s, (const bf16_t*) hin, R,
-                       (const bf16_t*) W, b, (bf16_t*) hout);
+    // persistent: one workgroup per CU (the weights fill most of its LDS)
+    static int ncu = 0;
+    if (ncu <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                  dev) != hipSuccess || ncu <= 0) ncu = 256;
+    }
+    const long nrb = (R + 31)/32;
+    const unsigned nblk = (unsigned) (nrb < ncu ? nrb : ncu);
+    hipLaunchKernelGGL(k_inv_hidden, dim3(nblk), dim3(256), 0, s,
+                       (const bf16_t*) hin, R, (const bf16_t*) W, b,
+                       (bf16_t*) hout);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
