@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
+#include <type_traits>
 #include "kernels.h"
 
 namespace {
@@ -902,17 +903,14 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
         *(bf16x8*) (ws + n*kInvWStride + k8) = *(const bf16x8*) (W + n*H + k8);
     }
     const long nrb = (R + 31)/32;
+    // rows past R (and past the last block) load row R - 1: a result row
+    // depends only on its own input row and those rows are never stored,
+    // so the loads need no branch
     auto load_a = [&](long rb, bf16x8* a) {
-        const long arow = rb*32 + (lane & 31);
-        const bool aok = rb < nrb && arow < R;
-        const bf16_t* ap = hin + ((long) d*R + (aok ? arow : 0))*H
-                         + 8*(lane >> 5);
+        const long arow = min(rb*32 + (lane & 31), R - 1);
+        const bf16_t* ap = hin + ((long) d*R + arow)*H + 8*(lane >> 5);
         #pragma unroll
-        for (int kk = 0; kk < KS; kk++) {
-            bf16x8 v = {};
-            if (aok) v = *(const bf16x8*) (ap + 16*kk);
-            a[kk] = v;
-        }
+        for (int kk = 0; kk < KS; kk++) a[kk] = *(const bf16x8*) (ap + 16*kk);
     };
     bf16x8 a[KS], an[KS];
     long rb = blockIdx.x;
@@ -921,6 +919,7 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
     for (; rb < nrb; rb += gridDim.x) {
         load_a(rb + gridDim.x, an);
         const long s0 = rb*32;
+        const bool full = s0 + 32 <= R;
         for (int ct = 0; ct < H/32; ct++) {
             const int n = 32*ct + (lane & 31);
             const bf16_t* wp = ws + n*kInvWStride + 8*(lane >> 5);
@@ -933,24 +932,35 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
             }
             const float bn = b[n];
             float (*dsb)[33] = ds[ct & 1];
-            if (d == 0) {
+            // a block inside R stores without a branch per row
+            const int mlim = full ? 32 : (int) (R - s0);
+            auto primal = [&](auto guarded) {
+                bf16_t* hp = hout + s0*H + n;
                 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
                     const float z = acc[r] + bn;
                     dsb[m][lane & 31] = dsilu(z);
-                    if (s0 + m < R) hout[(s0 + m)*H + n] = f2bf(silu(z));
+                    if (!guarded || m < mlim) hp[m*H] = f2bf(silu(z));
                 }
-            }
-            __syncthreads();
-            if (d > 0) {
+            };
+            auto tangent = [&](auto guarded) {
+                bf16_t* hp = hout + ((long) d*R + s0)*H + n;
                 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int m = 8*(r >> 2) + 4*(lane >> 5) + (r & 3);
-                    if (s0 + m < R)
-                        hout[((long) d*R + s0 + m)*H + n] =
-                            f2bf(dsb[m][lane & 31]*acc[r]);
+                    if (!guarded || m < mlim)
+                        hp[m*H] = f2bf(dsb[m][lane & 31]*acc[r]);
                 }
+            };
+            if (d == 0) {
+                if (full) primal(std::false_type{});
+                else primal(std::true_type{});
+            }
+            __syncthreads();
+            if (d > 0) {
+                if (full) tangent(std::false_type{});
+                else tangent(std::true_type{});
             }
         }
         #pragma unroll
