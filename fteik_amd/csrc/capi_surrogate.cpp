@@ -526,11 +526,14 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
                                         gs.d_p + gs.ob[l], hout, H, s) == 0;
                 std::swap(hin, hout);
             }
-            ok = ok && fteik_k_gemm_bf16(hin, H, 1, gs.d_pb + gs.oW5, 1, H,
-                                         d_yall, T, (int) (4*R), T, H,
+            // output layer written station-major (y^T = W5 h^T: output s
+            // of row q at d_yall[s*4R + q]), so the Gauss-Newton kernel's
+            // lanes, one row each, read consecutive words
+            ok = ok && fteik_k_gemm_bf16(gs.d_pb + gs.oW5, H, 1, hin, 1, H,
+                                         d_yall, 4*R, T, (int) (4*R), H,
                                          nullptr, 0, s) == 0;
         }
-        ok = ok && fteik_k_inv_gn(d_yall, R, T,
+        ok = ok && fteik_k_inv_gn(d_yall, R, (int) (4*R),
                                   gs.useNet ? gs.d_p + gs.ob5 : gs.d_zero,
                                   gs.d_geo, d_o, d_w, gl.lntf, starts, d_pos,
                                   d_mis, it < iterations ? 1 : 0, s) == 0;

@@ -987,7 +987,9 @@ __global__ void k_inv_act(const float* __restrict__ zall, long R, int H,
 
 // One damped Gauss-Newton step of the locator misfit
 // 1/2 sum_i w_i^2 (r_i - t0)^2, r = obs - T(p), t0 the weighted mean of r,
-// for every (event, start) row; y holds 4 blocks of R x ldy outputs.
+// for every (event, start) row; y holds the network outputs station-major:
+// output s of row r of block d (primal, tangents x, y, z) at
+// y[s*ldy + d*R + r], ldy >= 4R.
 __global__ void k_inv_gn(const float* __restrict__ y, long R, int ldy,
                          const float* __restrict__ b5, GeoArgs g,
                          const float* __restrict__ obs,
@@ -1004,11 +1006,11 @@ __global__ void k_inv_gn(const float* __restrict__ y, long R, int ldy,
         float sw = 0.0f, mr = 0.0f, mj[3] = {0.0f, 0.0f, 0.0f};
         for (int s = 0; s < g.ntf; s++) {
             float gr[3];
-            const float t = t0_ray(p, s, g, gr)
-                          + g.scale*(y[r*ldy + s] + b5[s]);
+            const float* ys = y + (long) s*ldy + r;
+            const float t = t0_ray(p, s, g, gr) + g.scale*(ys[0] + b5[s]);
             res[s] = o[s] - t;
             for (int d = 0; d < 3; d++)
-                J[s][d] = gr[d] + g.scale*y[(long) (d + 1)*R*ldy + r*ldy + s];
+                J[s][d] = gr[d] + g.scale*ys[(long) (d + 1)*R];
             const float ws = w[s];
             sw += ws; mr += ws*res[s];
             for (int d = 0; d < 3; d++) mj[d] += ws*J[s][d];
