@@ -1002,28 +1002,36 @@ __global__ void k_inv_gn(const float* __restrict__ y, long R, int ldy,
         const float* o = obs + e*lntf;
         const float* w = wt + e*lntf;
         float* p = pos + 3*r;
-        float res[64], J[64][3];
-        float sw = 0.0f, mr = 0.0f, mj[3] = {0.0f, 0.0f, 0.0f};
-        for (int s = 0; s < g.ntf; s++) {
+        // station s's residual and Jacobian row; formed again by the second
+        // pass (the same operations, so the same values) rather than kept in
+        // per-thread arrays, which would live in scratch memory
+        auto station = [&](int s, float& res, float* J) {
             float gr[3];
             const float* ys = y + (long) s*ldy + r;
             const float t = t0_ray(p, s, g, gr) + g.scale*(ys[0] + b5[s]);
-            res[s] = o[s] - t;
+            res = o[s] - t;
             for (int d = 0; d < 3; d++)
-                J[s][d] = gr[d] + g.scale*ys[(long) (d + 1)*R];
+                J[d] = gr[d] + g.scale*ys[(long) (d + 1)*R];
+        };
+        float sw = 0.0f, mr = 0.0f, mj[3] = {0.0f, 0.0f, 0.0f};
+        for (int s = 0; s < g.ntf; s++) {
+            float res, J[3];
+            station(s, res, J);
             const float ws = w[s];
-            sw += ws; mr += ws*res[s];
-            for (int d = 0; d < 3; d++) mj[d] += ws*J[s][d];
+            sw += ws; mr += ws*res;
+            for (int d = 0; d < 3; d++) mj[d] += ws*J[d];
         }
         const float inv = sw > 0.0f ? 1.0f/sw : 0.0f;
         mr *= inv;
         for (int d = 0; d < 3; d++) mj[d] *= inv;
         float A[3][3] = {}, bb[3] = {}, f = 0.0f;
         for (int s = 0; s < g.ntf; s++) {
+            float res, J[3];
+            station(s, res, J);
             const float w2 = w[s]*w[s];
-            const float rc = res[s] - mr;
+            const float rc = res - mr;
             float jc[3];
-            for (int d = 0; d < 3; d++) jc[d] = J[s][d] - mj[d];
+            for (int d = 0; d < 3; d++) jc[d] = J[d] - mj[d];
             f += 0.5f*w2*rc*rc;
             for (int a = 0; a < 3; a++) {
                 bb[a] += w2*jc[a]*rc;
