@@ -1083,17 +1083,26 @@ __device__ __forceinline__ unsigned long long cube_min(
     const float* w = wt + (long) e*lntf;
     const unsigned char* sk = skip + (long) e*lntf;
     const float sqrtHalf = 0.70710677f;
+    // the cell's times of every station (at most 64), all loads issued
+    // before the first is used; station indices past ntf are clamped, so
+    // the loads need no branch
+    float T[64];
+    #pragma unroll
+    for (int s = 0; s < 64; s++)
+        T[s] = tab[(long) min(s, g.ntf - 1)*ldgrd + c];
     float t0 = 0.0f;
-    for (int s = 0; s < g.ntf; s++) {
-        if (sk[s]) continue;
-        float res = xnorm[e]*(o[s] - tab[(long) s*ldgrd + c]);
+    #pragma unroll
+    for (int s = 0; s < 64; s++) {
+        if (s >= g.ntf || sk[s]) continue;
+        float res = xnorm[e]*(o[s] - T[s]);
         t0 = t0 + w[s]*res;
     }
     float obj = 0.0f;
-    for (int s = 0; s < g.ntf; s++) {
-        if (sk[s]) continue;
+    #pragma unroll
+    for (int s = 0; s < 64; s++) {
+        if (s >= g.ntf || sk[s]) continue;
         float ws = sqrtHalf*w[s];
-        float est = tab[(long) s*ldgrd + c] + t0;
+        float est = T[s] + t0;
         float res = ws*(o[s] - est);
         obj = obj + res*res;
     }
