@@ -81,7 +81,10 @@ bool inv_work(long n, long lntf, long R)
               dalloc(&gs.d_ibest, (size_t) nev) && dalloc(&gs.d_ifb, (size_t) nev) &&
               dalloc(&gs.d_pos, 3*(size_t) nr) && dalloc(&gs.d_mis, (size_t) nr) &&
               dalloc(&gs.d_zp, (size_t) nr*H) &&
-              dalloc(&gs.d_yall, 4*(size_t) nr*T) &&
+              // station-major outputs: whole 32-station tiles of rows
+              // padded to a multiple of 32 (fteik_k_inv_out)
+              dalloc(&gs.d_yall, (size_t) ((T + 31)/32*32)*
+                                 (size_t) ((4*nr + 31)/32*32)) &&
               dalloc(&gs.d_ha, 4*(size_t) nr*H) &&
               dalloc(&gs.d_hb, 4*(size_t) nr*H))) {
             free_inv_work();
@@ -509,10 +512,12 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
               h2d(d_xn, xn.data(), xn.size()*sizeof(float)) &&
               h2d(d_sk, sk.data(), sk.size());
     ok = ok && fteik_k_inv_start(d_pos, R, starts, s) == 0;
+    // network outputs station-major: output s of row q at d_yall[s*ldy + q]
+    const long ldy = (4*R + 31)/32*32;
     if (ok && !gs.useNet)
         // baseline proposals: network outputs and tangents all zero, so the
         // Gauss-Newton steps fit the straight-ray times alone
-        ok = fteik_k_fill_u64((unsigned long long *) d_yall, 2*R*T, 0ull,
+        ok = fteik_k_fill_u64((unsigned long long *) d_yall, T*ldy/2, 0ull,
                               s) == 0;
     for (int it = 0; it <= iterations && ok; it++) {
         if (gs.useNet) {
@@ -526,14 +531,10 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
                                         gs.d_p + gs.ob[l], hout, H, s) == 0;
                 std::swap(hin, hout);
             }
-            // output layer written station-major (y^T = W5 h^T: output s
-            // of row q at d_yall[s*4R + q]), so the Gauss-Newton kernel's
-            // lanes, one row each, read consecutive words
-            ok = ok && fteik_k_gemm_bf16(gs.d_pb + gs.oW5, H, 1, hin, 1, H,
-                                         d_yall, 4*R, T, (int) (4*R), H,
-                                         nullptr, 0, s) == 0;
+            ok = ok && fteik_k_inv_out(hin, 4*R, gs.d_pb + gs.oW5, T, d_yall,
+                                       ldy, s) == 0;
         }
-        ok = ok && fteik_k_inv_gn(d_yall, R, (int) (4*R),
+        ok = ok && fteik_k_inv_gn(d_yall, R, (int) ldy,
                                   gs.useNet ? gs.d_p + gs.ob5 : gs.d_zero,
                                   gs.d_geo, d_o, d_w, gl.lntf, starts, d_pos,
                                   d_mis, it < iterations ? 1 : 0, s) == 0;

@@ -280,6 +280,12 @@ int fteik_k_sur_geo_size(void);
 int fteik_k_inv_start(float* pos, long R, int starts, hipStream_t stream);
 int fteik_k_inv_hidden(const void* hin, long R, const void* W,
                        const float* b, void* hout, int H, hipStream_t s);
+// Output layer of the inversion network, station-major: y[s*ldy + q] =
+// sum_k W5[s][k] h[q][k] for rows q < NR of h (NR x 256 BF16) and stations
+// s < T <= 64.  y must hold ((T + 31)/32)*32 rows of ldy >= NR floats, ldy
+// a multiple of 32: whole 32 x 32 tiles are stored.
+int fteik_k_inv_out(const void* hin, long NR, const void* W5, int T,
+                    float* y, long ldy, hipStream_t s);
 int fteik_k_inv_layer1(const float* pos, long R, const float* W1,
                        const float* b1, float* zp, void* h, int H,
                        hipStream_t stream);

@@ -968,6 +968,58 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
     }
 }
 
+// Output layer of the inversion network (no bias: the Gauss-Newton kernel
+// adds it), station-major so that the Gauss-Newton kernel's lanes, one row
+// each, read consecutive words: y[s*ldy + q] = sum_k W5[s][k] h[q][k].  The
+// weights (T <= 64 rows, padded with zero rows to 64 and to 264 elements
+// per row) sit in LDS as the MFMA's A operand; each wave holds 32 rows of h
+// over all of K as its B operand, so a result tile is 32 stations x 32 rows
+// and consecutive lanes store consecutive rows.  Whole tiles are stored
+// (y is allocated to ((T + 31)/32)*32 station rows and ldy >= NR rounded up
+// to 32), so the stores need no branch; rows past NR load row NR - 1.
+__global__ __launch_bounds__(256)
+void k_inv_out(const bf16_t* __restrict__ hin, long NR,
+               const bf16_t* __restrict__ W5, int T, float* __restrict__ y,
+               long ldy)
+{
+    constexpr int H = 256, KS = H/16;
+    __shared__ __attribute__((aligned(16))) bf16_t ws[64*kInvWStride];
+    for (int c = threadIdx.x; c < 64*H/8; c += 256) {
+        const int n = c >> 5, k8 = (c & 31)*8;
+        bf16x8 v = {};
+        if (n < T) v = *(const bf16x8*) (W5 + n*H + k8);
+        *(bf16x8*) (ws + n*kInvWStride + k8) = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nst = (T + 31)/32;
+    const long nblk = (NR + 31)/32;
+    for (long bq = (long) blockIdx.x*4 + w; bq < nblk;
+         bq += (long) gridDim.x*4) {
+        const long q = min(bq*32 + (lane & 31), NR - 1);
+        const bf16_t* hp = hin + q*H + 8*(lane >> 5);
+        bf16x8 bh[KS];
+        #pragma unroll
+        for (int kk = 0; kk < KS; kk++) bh[kk] = *(const bf16x8*) (hp + 16*kk);
+        for (int st = 0; st < nst; st++) {
+            const bf16_t* wp = ws + (32*st + (lane & 31))*kInvWStride
+                             + 8*(lane >> 5);
+            f32x16 acc = {};
+            #pragma unroll
+            for (int kk = 0; kk < KS; kk++) {
+                const bf16x8 av = *(const bf16x8*) (wp + 16*kk);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bh[kk], acc,
+                                                              0, 0, 0);
+            }
+            float* yp = y + (long) (32*st + 4*(lane >> 5))*ldy + bq*32
+                      + (lane & 31);
+            #pragma unroll
+            for (int r = 0; r < 16; r++)
+                yp[(long) (8*(r >> 2) + (r & 3))*ldy] = acc[r];
+        }
+    }
+}
+
 // After a hidden GEMM over all 4R rows (no bias): primal h = silu(z + b),
 // tangents h_d = silu'(z + b) * z_d.
 __global__ void k_inv_act(const float* __restrict__ zall, long R, int H,
@@ -1351,6 +1403,24 @@ int fteik_k_inv_hidden(const void* hin, long R, const void* W,
     hipLaunchKernelGGL(k_inv_hidden, dim3(nblk), dim3(256), 0, s,
                        (const bf16_t*) hin, R, (const bf16_t*) W, b,
                        (bf16_t*) hout);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int fteik_k_inv_out(const void* hin, long NR, const void* W5, int T,
+                    float* y, long ldy, hipStream_t s)
+{
+    if (T < 1 || T > 64 || NR < 1 || ldy < NR || ldy % 32 != 0) return 2;
+    static int ncu = 0;
+    if (ncu <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                  dev) != hipSuccess || ncu <= 0) ncu = 256;
+    }
+    const long nwg = (NR + 127)/128;          // 4 waves of 32 rows each
+    const unsigned nblk = (unsigned) (nwg < 4L*ncu ? nwg : 4L*ncu);
+    hipLaunchKernelGGL(k_inv_out, dim3(nblk), dim3(256), 0, s,
+                       (const bf16_t*) hin, NR, (const bf16_t*) W5, T, y, ldy);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
