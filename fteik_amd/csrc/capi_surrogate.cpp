@@ -48,7 +48,7 @@ struct Surrogate {
     unsigned char *d_isk = nullptr;
     unsigned long long *d_ibest = nullptr;
     int *d_ifb = nullptr;
-    float *d_pos = nullptr, *d_mis = nullptr, *d_zp = nullptr,
+    float *d_pos = nullptr, *d_mis = nullptr,
           *d_yall = nullptr;
     unsigned short *d_ha = nullptr, *d_hb = nullptr;
     // proposals from the network (default) or from the straight-ray
@@ -62,7 +62,7 @@ void free_inv_work()
 {
     dfree(gs.d_io); dfree(gs.d_iw); dfree(gs.d_ixn); dfree(gs.d_isk);
     dfree(gs.d_ibest); dfree(gs.d_ifb);
-    dfree(gs.d_pos); dfree(gs.d_mis); dfree(gs.d_zp);
+    dfree(gs.d_pos); dfree(gs.d_mis);
     dfree(gs.d_yall); dfree(gs.d_ha); dfree(gs.d_hb);
     gs.invEvents = gs.invRows = 0;
 }
@@ -80,7 +80,6 @@ bool inv_work(long n, long lntf, long R)
               dalloc(&gs.d_isk, (size_t) ne) && dalloc(&gs.d_ixn, (size_t) nev) &&
               dalloc(&gs.d_ibest, (size_t) nev) && dalloc(&gs.d_ifb, (size_t) nev) &&
               dalloc(&gs.d_pos, 3*(size_t) nr) && dalloc(&gs.d_mis, (size_t) nr) &&
-              dalloc(&gs.d_zp, (size_t) nr*H) &&
               // station-major outputs: whole 32-station tiles of rows
               // padded to a multiple of 32 (fteik_k_inv_out)
               dalloc(&gs.d_yall, (size_t) ((T + 31)/32*32)*
@@ -501,7 +500,7 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
         return;
     }
     float *d_o = gs.d_io, *d_w = gs.d_iw, *d_xn = gs.d_ixn, *d_pos = gs.d_pos,
-          *d_mis = gs.d_mis, *d_zp = gs.d_zp,
+          *d_mis = gs.d_mis,
           *d_yall = gs.d_yall;
     unsigned short *d_ha = gs.d_ha, *d_hb = gs.d_hb;
     unsigned char *d_sk = gs.d_isk;
@@ -523,7 +522,7 @@ void locate_locateEventsSurrogate(int n, const int *evnmbr, int starts,
         if (gs.useNet) {
             // forward with tangents over 4R rows
             ok = fteik_k_inv_layer1(d_pos, R, gs.d_p + gs.oW1,
-                                    gs.d_p + gs.ob1, d_zp, d_ha, H, s) == 0;
+                                    gs.d_p + gs.ob1, nullptr, d_ha, H, s) == 0;
             unsigned short *hin = d_ha, *hout = d_hb;
             for (int l = 0; l < 3 && ok; l++) {
                 // fused GEMM + activation (the GEMM then k_inv_act, bitwise)

@@ -854,6 +854,7 @@ __global__ void k_samples(const float* __restrict__ tab, long ldgrd, GeoArgs g,
 
 // ------------------------------------------------------------ inversion
 // Rows are laid out as 4 blocks of R: primal, then tangents along x, y, z.
+// zp (the primal pre-activations) may be null.
 __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
                              const float* __restrict__ W1,
                              const float* __restrict__ b1,
@@ -867,7 +868,7 @@ __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
         const float z = W1[3*j]*p[0] + W1[3*j + 1]*p[1] + W1[3*j + 2]*p[2]
                       + b1[j];
         const float ds = dsilu(z);
-        zp[i] = z;
+        if (zp) zp[i] = z;
         h[i] = f2bf(silu(z));
         for (int d = 0; d < 3; d++)
             h[(long) (d + 1)*R*H + i] = f2bf(ds*W1[3*j + d]);
