@@ -121,7 +121,7 @@ void k_locate_one(const float* __restrict__ tab, int ntf, int ngrd, long ldgrd,
 // event).  blockIdx.y selects a group of epg consecutive events, so the
 // launch can be cut into workgroups that fill whole rounds of the GPU
 // (event_groups).
-template <int NTF, int CPT>
+template <int NTF, int CPT, bool WGMIN>
 __global__ __launch_bounds__(TL)
 void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
                     const float* __restrict__ obs, const float* __restrict__ wt,
@@ -129,6 +129,7 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
                     const float* __restrict__ xnorm, int lntf, int nev,
                     int epg, unsigned long long* __restrict__ best)
 {
+    __shared__ unsigned long long s_key[WGMIN ? TL/64 : 1];
     const long base = ((long) blockIdx.x*TL*CPT) + threadIdx.x;
     // events [e0, e1) of this workgroup's event group
     const int e0 = (int) blockIdx.y*epg, e1 = min(nev, e0 + epg);
@@ -205,7 +206,26 @@ void k_locate_batch(const float* __restrict__ tab, int ngrd, long ldgrd,
             key = (c < ngrd && k < key) ? k : key;
         }
         key = wave_min(key);
-        if ((threadIdx.x & 63) == 0 && key != ~0ull) atomicMin(best + e, key);
+        if (WGMIN) {
+            // few events: every workgroup would post to the same few
+            // words, and atomics on one word serialize (one event over
+            // 10,936 waves took 0.77 ms).  One post per workgroup, and none
+            // when the event's minimum is already lower (the word only
+            // decreases, so a stale read can only cause a needless post).
+            if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = key;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned long long k = s_key[0];
+                #pragma unroll
+                for (int q = 1; q < TL/64; q++) k = (s_key[q] < k) ? s_key[q] : k;
+                if (k < __hip_atomic_load(best + e, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT))
+                    atomicMin(best + e, k);
+            }
+            __syncthreads();
+        } else if ((threadIdx.x & 63) == 0 && key != ~0ull) {
+            atomicMin(best + e, key);
+        }
     }
 }
 
@@ -297,8 +317,13 @@ int fteik_k_locate_batch(const float* tables, int ntf, int ngrd, long ldgrd,
     // events: 2 -> 58.7k, 3 -> 61.7k, 4 -> 53.6k events/s (4 costs occupancy)
     constexpr int CPT = 3;
     int nblk = (int) ((ngrd + (long) TL*CPT - 1)/((long) TL*CPT));
+    // few events: the variant that posts one key per workgroup (see the
+    // kernel's key posting)
+    const bool few = nev <= 8;
     const void* fn = nullptr;
-#define LB(N) case N: fn = (const void*) k_locate_batch<N, CPT>; break;
+#define LB(N) case N: fn = few ? (const void*) k_locate_batch<N, CPT, true> \
+                               : (const void*) k_locate_batch<N, CPT, false>; \
+                      break;
     switch (ntf) {
         LB(1) LB(2) LB(3) LB(4) LB(5) LB(6) LB(7) LB(8) LB(10) LB(12) LB(16)
         LB(20) LB(24) LB(32) LB(40) LB(48) LB(56) LB(64)
