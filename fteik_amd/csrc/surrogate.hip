@@ -921,16 +921,26 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
         load_a(rb + gridDim.x, an);
         const long s0 = rb*32;
         const bool full = s0 + 32 <= R;
-        for (int ct = 0; ct < H/32; ct++) {
-            const int n = 32*ct + (lane & 31);
-            const bf16_t* wp = ws + n*kInvWStride + 8*(lane >> 5);
-            f32x16 acc = {};
+        // the next column tile's products are issued before this tile's
+        // epilogue, so the matrix units run while the epilogue does
+        auto tile = [&](int ct, f32x16& acc) {
+            const bf16_t* wp = ws + (32*ct + (lane & 31))*kInvWStride
+                             + 8*(lane >> 5);
+            acc = f32x16{};
             #pragma unroll
             for (int kk = 0; kk < KS; kk++) {
                 const bf16x8 bv = *(const bf16x8*) (wp + 16*kk);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk], bv, acc,
                                                               0, 0, 0);
             }
+        };
+        f32x16 accs[2];
+        tile(0, accs[0]);
+        #pragma unroll
+        for (int ct = 0; ct < H/32; ct++) {
+            const int n = 32*ct + (lane & 31);
+            if (ct + 1 < H/32) tile(ct + 1, accs[(ct + 1) & 1]);
+            const f32x16& acc = accs[ct & 1];
             const float bn = b[n];
             float (*dsb)[33] = ds[ct & 1];
             // a block inside R stores without a branch per row
