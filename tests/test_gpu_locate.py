@@ -110,6 +110,32 @@ def test_fresh_and_batched_match_oracle(nsta):
     L.locate_finalize()
 
 
+@pytest.mark.parametrize("nev", [1, 2, 8, 9])
+def test_few_events_match_oracle(nev):
+    """Batched searches of at most 8 events post one key per workgroup
+    (a kernel variant of their own); 9 events take the many-event kernel.
+    Both equal the oracle bit for bit (locate.F90:620-744, MINLOC :122)."""
+    nsta = 12
+    tables, picks, wts, ngrd = _random_case(21, 41, 37, 29, nsta, nev)
+    L = _setup(nev, nsta, ngrd, tables, picks, wts)
+    ierr = c_int(1)
+    L.locate_setReferenceCompatibility(0, byref(ierr))
+    loc = orc.Locator(nev, nsta, ngrd)
+    for s in range(nsta):
+        loc.set_table(s + 1, tables[s])
+    ev = np.arange(1, nev + 1, dtype=np.int32)
+    opt = np.zeros(nev, dtype=np.int32)
+    t0 = np.zeros(nev)
+    ob = np.zeros(nev)
+    L.locate_locateEvents(nev, _i(ev), _i(opt), _d(t0), _d(ob), byref(ierr))
+    assert ierr.value == 0
+    for e in range(nev):
+        loc.set_observation(e + 1, np.arange(1, nsta + 1), picks[e], wts[e])
+        assert (opt[e], t0[e], ob[e]) == loc.locate(e + 1, False, False)
+    L.locate_setReferenceCompatibility(1, byref(ierr))
+    L.locate_finalize()
+
+
 def test_table_count_without_specialised_kernel():
     """ntf = 9 has no register-resident batch kernel: one pass per event."""
     nsta, nev = 9, 6
