@@ -883,30 +883,31 @@ __global__ void k_inv_layer1(const float* __restrict__ pos, long R,
 // k_gemm followed by k_inv_act computes, without the FP32 round trip
 // through memory.  Persistent: each workgroup copies W into LDS once (rows
 // padded to 264 elements so the 32 rows an MFMA operand load touches fall
-// in different banks) and then takes 32-row blocks in turn; wave d computes
-// block d with its A rows held in registers over the whole K, the next
-// block's rows loading while the current one computes, and the primal wave
-// hands silu' to the tangent waves through LDS one 32-column tile at a
-// time.  Streaming W from L2 for every wave instead read 1.3 GB of L2 per
-// launch at 80,000 rows.
+// in different banks) and then takes 32-row blocks in turn.  Eight waves,
+// two per SIMD (the weights fill most of the LDS, so a CU holds one
+// workgroup): waves d and d + 4 compute block d, the first the column tiles
+// 0-3 and the second 4-7, each with its A rows held in registers over the
+// whole K and the next block's rows loading while it computes; a tile's
+// MFMAs are issued before the previous tile's epilogue.  The primal waves
+// hand silu' to the tangent waves through LDS one 32-column tile at a time.
+// Streaming W from L2 for every wave instead read 1.3 GB of L2 per launch at
+// 80,000 rows.
 constexpr int kInvWStride = 264;
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(512)
 void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
                   const bf16_t* __restrict__ W, const float* __restrict__ b,
                   bf16_t* __restrict__ hout)
 {
     constexpr int H = 256, KS = H/16;
     __shared__ __attribute__((aligned(16))) bf16_t ws[H*kInvWStride];
-    __shared__ float ds[2][32][33];
-    const int lane = threadIdx.x & 63, d = threadIdx.x >> 6;
-    for (int c = threadIdx.x; c < H*H/8; c += 256) {
+    __shared__ float ds[2][2][32][33];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int d = w & 3, half = w >> 2;
+    for (int c = threadIdx.x; c < H*H/8; c += 512) {
         const int n = c >> 5, k8 = (c & 31)*8;
         *(bf16x8*) (ws + n*kInvWStride + k8) = *(const bf16x8*) (W + n*H + k8);
     }
     const long nrb = (R + 31)/32;
-    // rows past R (and past the last block) load row R - 1: a result row
-    // depends only on its own input row and those rows are never stored,
-    // so the loads need no branch
     auto load_a = [&](long rb, bf16x8* a) {
         const long arow = min(rb*32 + (lane & 31), R - 1);
         const bf16_t* ap = hin + ((long) d*R + arow)*H + 8*(lane >> 5);
@@ -921,8 +922,6 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
         load_a(rb + gridDim.x, an);
         const long s0 = rb*32;
         const bool full = s0 + 32 <= R;
-        // the next column tile's products are issued before this tile's
-        // epilogue, so the matrix units run while the epilogue does
         auto tile = [&](int ct, f32x16& acc) {
             const bf16_t* wp = ws + (32*ct + (lane & 31))*kInvWStride
                              + 8*(lane >> 5);
@@ -935,15 +934,15 @@ void k_inv_hidden(const bf16_t* __restrict__ hin, long R,
             }
         };
         f32x16 accs[2];
-        tile(0, accs[0]);
+        tile(4*half, accs[0]);
         #pragma unroll
-        for (int ct = 0; ct < H/32; ct++) {
+        for (int cl = 0; cl < 4; cl++) {
+            const int ct = 4*half + cl;
             const int n = 32*ct + (lane & 31);
-            if (ct + 1 < H/32) tile(ct + 1, accs[(ct + 1) & 1]);
-            const f32x16& acc = accs[ct & 1];
+            if (cl + 1 < 4) tile(ct + 1, accs[(cl + 1) & 1]);
+            const f32x16& acc = accs[cl & 1];
             const float bn = b[n];
-            float (*dsb)[33] = ds[ct & 1];
-            // a block inside R stores without a branch per row
+            float (*dsb)[33] = ds[cl & 1][half];
             const int mlim = full ? 32 : (int) (R - s0);
             auto primal = [&](auto guarded) {
                 bf16_t* hp = hout + s0*H + n;
@@ -1411,7 +1410,7 @@ int fteik_k_inv_hidden(const void* hin, long R, const void* W,
     }
     const long nrb = (R + 31)/32;
     const unsigned nblk = (unsigned) (nrb < ncu ? nrb : ncu);
-    hipLaunchKernelGGL(k_inv_hidden, dim3(nblk), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_inv_hidden, dim3(nblk), dim3(512), 0, s,
                        (const bf16_t*) hin, R, (const bf16_t*) W, b,
                        (bf16_t*) hout);
     return hipGetLastError() == hipSuccess ? 0 : 1;
